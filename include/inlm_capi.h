@@ -1,0 +1,184 @@
+/*
+ * inlm_capi.h -- C ABI of the MI355X-native iterNLM hot path.
+ *
+ * This is the drop-in boundary for the reference's GPU denoiser class
+ * denoise::IterativeNLM_GPU (Denoiser/iternlm_gpu.h:20-56 in
+ * iternlm/microtomodenoise iterNLM_v0.3).  Each entry point names the
+ * reference member it replaces.  Plain C types only: no C++ and no torch
+ * types cross this boundary.
+ *
+ * Conventions
+ *   - Volumes are float32, x fastest: idx = z*nx*ny + y*nx + x, with
+ *     shape = {nx, ny, nz} (the reference's shape[0] = TIFF width).
+ *   - Host buffers belong to the caller; a handle owns all device state.
+ *   - A handle is single-threaded.  Different handles are independent: there
+ *     is no global mutable state, all constants travel as kernel arguments
+ *     (the reference keeps them in __constant__ globals, which made it
+ *     non-reentrant, gpu_constants.cuh:13-22).
+ *   - Every call is synchronous at return; work runs on one HIP stream of
+ *     default priority owned by the handle.
+ *   - Errors are status codes; inlm_last_error() gives the message.
+ */
+#ifndef INLM_CAPI_H
+#define INLM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INLM_API __attribute__((visibility("default")))
+
+/* Status codes.  NO_DEVICE mirrors configure_device's -1 (no GPU, the caller
+ * falls back to the CPU path, main.cpp:317) and HIP_ERROR its -2
+ * (iternlm_gpu.cu:1608-1618). */
+typedef enum {
+    INLM_OK = 0,
+    INLM_NO_DEVICE = -1,
+    INLM_HIP_ERROR = -2,
+    INLM_BAD_ARG = -3,
+    INLM_OOM = -4
+} inlm_status;
+
+/* NoiseLevel.mode of denoise_parameters.h:23 ("global"/"auto",
+ * "z-adaptive", "manual"). */
+typedef enum { INLM_NOISE_GLOBAL = 0, INLM_NOISE_ZADAPTIVE = 1, INLM_NOISE_MANUAL = 2 } inlm_noise_mode;
+
+/* Arithmetic form of the filter kernel.
+ *   FP32      -- parity form: operation-faithful to the reference CPU path
+ *                (tap order, unfused multiply/add, IEEE division), bit-exact
+ *                against it.
+ *   FP32_FAST -- FP32 storage, fused multiply-add stencil and reciprocal-based
+ *                Pade; differs from FP32 by rounding only.
+ *   BF16/FP16 -- R and P stored in 16 bits after the affine map
+ *                (v - shift) * scale; differences, sums and accumulators in
+ *                FP32, normalisation in FP64. */
+typedef enum { INLM_PREC_FP32 = 0, INLM_PREC_FP32_FAST = 1, INLM_PREC_BF16 = 2, INLM_PREC_FP16 = 3 } inlm_precision;
+
+/* Weight function: the reference CPU's Pade [3/3] with the -3.56648 cutoff
+ * (Denoiser/iternlm_cpu.h:125-126) or the reference GPU's expf
+ * (iternlm_gpu.cu:150). */
+typedef enum { INLM_WEIGHT_PADE = 0, INLM_WEIGHT_EXP = 1 } inlm_weight_fn;
+
+/* POD mirror of protocol::DenoiseParameters (denoise_parameters.h:9-74),
+ * restricted to the fields the hot path reads, plus the build's own
+ * precision / weight-function / sampler-seed choices. */
+typedef struct inlm_params {
+    float alpha;                   /* 0.5 */
+    int32_t maxiterations;         /* 4 */
+    int32_t nslices;               /* 11 */
+    int32_t radius_searchspace[3]; /* 10,10,10 */
+    int32_t radius_patchspace[3];  /* 1,1,1 */
+    float beta;                    /* 1 */
+    float z_anisotropy;            /* 1 */
+    /* NoiseLevel */
+    int32_t noise_mode;            /* inlm_noise_mode, z-adaptive */
+    int32_t zwindow;               /* 100 */
+    float sigma[2];                /* manual mode */
+    int64_t n_samples;             /* 1e5 */
+    int32_t patchsize;             /* 15 */
+    float stds_from_mean;          /* 1 */
+    int32_t continuous_estimate;   /* 0 */
+    uint64_t sample_seed;          /* seeds the sample positions (the reference seeds from the clock) */
+    /* CPU.blockwise: selects which reference CPU dispatch the weight table
+     * follows (iternlm_cpu.cpp:97-114 vs :529-548). */
+    int32_t cpu_blockwise;         /* 1 */
+    /* build-specific */
+    int32_t precision;             /* inlm_precision, FP32 */
+    int32_t weight_fn;             /* inlm_weight_fn, PADE */
+    int32_t kernel_select;         /* 0: tiled kernel where one covers the patch, 1: per-voxel kernel */
+    int32_t reserved[7];
+} inlm_params;
+
+typedef struct inlm_handle inlm_handle;
+
+/* Fills the reference defaults (denoise_parameters.h:11-48). */
+INLM_API void inlm_default_params(inlm_params *p);
+
+/* Number of visible GPUs (0 when none). */
+INLM_API int32_t inlm_device_count(void);
+
+/* Replaces IterativeNLM_GPU::configure_device (iternlm_gpu.cu:1440-1623):
+ * builds the search/patch tables on the padded shape, allocates the padded
+ * original, the padded previous iterate and the result on `device`.
+ * Returns NO_DEVICE when no GPU is visible. */
+INLM_API inlm_status inlm_create(const inlm_params *p, const int32_t shape[3], int32_t device, inlm_handle **out);
+
+/* Replaces IterativeNLM_GPU::free_device (iternlm_gpu.cu:1625-1649). */
+INLM_API inlm_status inlm_destroy(inlm_handle *h);
+
+/* Last error message of a handle (or of the last failed inlm_create when h
+ * is NULL). */
+INLM_API const char *inlm_last_error(const inlm_handle *h);
+
+/* Replaces IterativeNLM_GPU::set_sigma (iternlm_gpu.cu:2341-2355): one
+ * filter strength per slice (nz values). */
+INLM_API inlm_status inlm_set_sigma(inlm_handle *h, const float *sigma, int32_t nz);
+
+/* Loads the original noisy volume (nx*ny*nz floats) from host memory or, with
+ * is_device != 0, from device memory on the handle's GPU; reflect-pads it
+ * into the resident original and makes it the previous iterate.  This is
+ * the first half of prepare_iteration1 (iternlm_gpu.cu:2117-2152). */
+INLM_API inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device);
+
+/* Replaces IterativeNLM_GPU::Run_GaussianNoise (iternlm_gpu.cu:1651-1728):
+ * iteration `iter` (1-based).  At iter == 1 a non-NULL host `raw` is loaded
+ * first (as the reference uploads instack); at iter > 1 the previous result
+ * is reflect-padded into the previous iterate (prepare_nextiteration,
+ * iternlm_gpu.cu:2230-2243).  Then one filter pass writes the result. */
+INLM_API inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw);
+
+/* Replaces IterativeNLM_GPU::get_result (iternlm_gpu.cu:2356-2368): copies
+ * the last result (nx*ny*nz floats) to host memory, or to device memory when
+ * is_device != 0. */
+INLM_API inlm_status inlm_get_result(inlm_handle *h, float *out, int32_t is_device);
+
+/* Sample positions for the noise estimator: the distribution of
+ * NoiseLevel::drawsamples (Noise/noiselevel.cpp:192-234) drawn from a
+ * SplitMix64 stream seeded with `seed`, sorted.  Host-only; needs no GPU. */
+INLM_API inlm_status inlm_draw_samples(const int32_t shape[3], int64_t n, int32_t patchsize, uint64_t seed,
+                                       int64_t *out);
+
+/* Per-sample 15x15 (patchsize^2) in-slice standard deviation on the GPU
+ * (NoiseLevel::get_patchstd, noiselevel.cpp:235-283), one wavefront per
+ * sample, FP64 accumulation.  which = 0: the original volume; 1: the last
+ * result. */
+INLM_API inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, double *out);
+
+/* Noise level per slice (NoiseLevel::get_noiselevel, noiselevel.cpp:9-95):
+ * patch std on the GPU, two-cluster EM in FP64 on the host, global or
+ * z-adaptive, sigma = mean + stds_from_mean * std.  Writes nz floats. */
+INLM_API inlm_status inlm_estimate_noise(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n,
+                                         float *sigma_out);
+
+/* The whole schedule of main.cpp:325-545 on a resident volume: sigma0 from
+ * the original, iteration 1, sigma1 re-estimated on its output with the same
+ * samples and blended alpha*s0 + (1-alpha)*s1, iterations 2..maxiterations
+ * (re-estimating each iteration with continuous_estimate).  raw/out are host
+ * pointers (raw may be NULL when inlm_load_raw was called).  sigma_log, when
+ * not NULL, receives the sigma list used by each iteration
+ * (maxiterations*nz floats). */
+INLM_API inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, float *sigma_log);
+
+/* Host-only query of the tables a handle with these parameters would use on
+ * a volume of `shape`: search offsets and patch taps as (dx,dy,dz) triples in
+ * the reference order (setup_searchspace / setup_patchspace,
+ * iternlm_prepare.cpp:102-203) and the patch weights the reference CPU path
+ * applies (literal tables of iternlm_cpu_kernels.cpp or setup_distweight,
+ * iternlm_prepare.cpp:204-263).  Writes at most `cap` entries per table and
+ * returns the true sizes.  Needs no GPU. */
+INLM_API inlm_status inlm_tables(const inlm_params *p, const int32_t shape[3], int32_t cap, int32_t *n_search,
+                                 int32_t *search_dxyz, int32_t *n_patch, int32_t *patch_dxyz, float *weights);
+
+/* Kernel time of the last filter pass in milliseconds (HIP events recorded
+ * on the handle's stream around the filter launch). */
+INLM_API float inlm_last_filter_ms(const inlm_handle *h);
+
+/* Table sizes of a handle: offsets visited and patch taps. */
+INLM_API inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n_patch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INLM_CAPI_H */

@@ -1,0 +1,136 @@
+"""ctypes binding of ``libinlm.so`` (the C ABI declared in include/inlm_capi.h).
+
+The library is loaded from this package directory.  There is no fallback:
+when the library is missing, importing the binding raises, so no call can
+silently run anything other than the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libinlm.so")
+
+INLM_OK, INLM_NO_DEVICE, INLM_HIP_ERROR, INLM_BAD_ARG, INLM_OOM = 0, -1, -2, -3, -4
+NOISE_GLOBAL, NOISE_ZADAPTIVE, NOISE_MANUAL = 0, 1, 2
+PREC_FP32, PREC_FP32_FAST, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
+WEIGHT_PADE, WEIGHT_EXP = 0, 1
+
+STATUS_NAMES = {0: "OK", -1: "NO_DEVICE", -2: "HIP_ERROR", -3: "BAD_ARG", -4: "OOM"}
+
+# Every symbol include/inlm_capi.h declares.
+EXPORTS = [
+    "inlm_default_params", "inlm_device_count", "inlm_create", "inlm_destroy", "inlm_last_error",
+    "inlm_set_sigma", "inlm_load_raw", "inlm_run_iteration", "inlm_get_result", "inlm_draw_samples",
+    "inlm_patchstd", "inlm_estimate_noise", "inlm_denoise", "inlm_tables", "inlm_last_filter_ms",
+    "inlm_table_sizes",
+]
+
+
+class InlmParams(C.Structure):
+    """Mirror of ``inlm_params`` (include/inlm_capi.h)."""
+    _fields_ = [
+        ("alpha", C.c_float),
+        ("maxiterations", C.c_int32),
+        ("nslices", C.c_int32),
+        ("radius_searchspace", C.c_int32 * 3),
+        ("radius_patchspace", C.c_int32 * 3),
+        ("beta", C.c_float),
+        ("z_anisotropy", C.c_float),
+        ("noise_mode", C.c_int32),
+        ("zwindow", C.c_int32),
+        ("sigma", C.c_float * 2),
+        ("n_samples", C.c_int64),
+        ("patchsize", C.c_int32),
+        ("stds_from_mean", C.c_float),
+        ("continuous_estimate", C.c_int32),
+        ("sample_seed", C.c_uint64),
+        ("cpu_blockwise", C.c_int32),
+        ("precision", C.c_int32),
+        ("weight_fn", C.c_int32),
+        ("kernel_select", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class InlmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m microtomodenoise_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        H = C.c_void_p
+        f32p, f64p, i64p, i32p = (C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int32))
+        L.inlm_default_params.argtypes = [C.POINTER(InlmParams)]
+        L.inlm_default_params.restype = None
+        L.inlm_device_count.restype = C.c_int32
+        L.inlm_create.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, C.POINTER(H)]
+        L.inlm_destroy.argtypes = [H]
+        L.inlm_last_error.argtypes = [H]
+        L.inlm_last_error.restype = C.c_char_p
+        L.inlm_set_sigma.argtypes = [H, f32p, C.c_int32]
+        L.inlm_load_raw.argtypes = [H, C.c_void_p, C.c_int32]
+        L.inlm_run_iteration.argtypes = [H, C.c_int32, f32p]
+        L.inlm_get_result.argtypes = [H, C.c_void_p, C.c_int32]
+        L.inlm_draw_samples.argtypes = [i32p, C.c_int64, C.c_int32, C.c_uint64, i64p]
+        L.inlm_patchstd.argtypes = [H, C.c_int32, i64p, C.c_int64, f64p]
+        L.inlm_estimate_noise.argtypes = [H, C.c_int32, i64p, C.c_int64, f32p]
+        L.inlm_denoise.argtypes = [H, f32p, f32p, f32p]
+        L.inlm_last_filter_ms.argtypes = [H]
+        L.inlm_last_filter_ms.restype = C.c_float
+        L.inlm_table_sizes.argtypes = [H, i32p, i32p]
+        L.inlm_tables.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, i32p, i32p, i32p, i32p, f32p]
+        _LIB = L
+    return _LIB
+
+
+def ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def check(status: int, handle=None):
+    if status != INLM_OK:
+        msg = lib().inlm_last_error(handle)
+        raise InlmError(status, msg.decode() if msg else "")
+
+
+def default_params() -> InlmParams:
+    p = InlmParams()
+    lib().inlm_default_params(C.byref(p))
+    return p
+
+
+def draw_samples(shape_xyz, n: int, patchsize: int = 15, seed: int = 0) -> np.ndarray:
+    """Sample positions of the noise estimator (host-only, no GPU needed)."""
+    out = np.zeros(n, np.int64)
+    sh = np.asarray(shape_xyz, np.int32)
+    check(lib().inlm_draw_samples(ptr(sh, C.c_int32), n, patchsize, seed, ptr(out, C.c_int64)))
+    return out
+
+
+def tables(params: InlmParams, shape_xyz):
+    """Host-only table query (inlm_tables): search offsets, patch taps and
+    weights as the library builds them."""
+    cap = 16384
+    sd = np.zeros(3 * cap, np.int32)
+    pd = np.zeros(3 * cap, np.int32)
+    w = np.zeros(cap, np.float32)
+    ns, npt = C.c_int32(), C.c_int32()
+    sh = np.asarray(shape_xyz, np.int32)
+    check(lib().inlm_tables(C.byref(params), ptr(sh, C.c_int32), cap, C.byref(ns), ptr(sd, C.c_int32),
+                            C.byref(npt), ptr(pd, C.c_int32), ptr(w, C.c_float)))
+    return (sd[: 3 * ns.value].reshape(-1, 3).copy(), pd[: 3 * npt.value].reshape(-1, 3).copy(),
+            w[: npt.value].copy())

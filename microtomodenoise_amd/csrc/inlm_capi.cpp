@@ -1,0 +1,485 @@
+// C ABI of the iterNLM hot path: one handle per volume, owning the resident
+// padded original, the padded previous iterate, the result and the tables.
+#include "inlm_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "inlm_kernels.hpp"
+#include "inlm_tables.hpp"
+
+namespace {
+
+std::string g_create_error;  // message of the last failed inlm_create
+
+}  // namespace
+
+struct inlm_handle {
+    inlm_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    inlm::Geometry geo{};
+    inlm::FilterGeom fg{};
+    std::vector<inlm::SearchEntry> search;
+    std::vector<int> tap_dxyz;
+    inlm::TapTable taps{};
+    int rp_tiled = 0;
+    size_t pad_elems = 0;  // allocated elements of each padded volume (>= padded voxels)
+    float *R_pad = nullptr, *P_pad = nullptr, *result = nullptr, *sigma = nullptr;
+    inlm::SearchEntry *search_dev = nullptr;
+    int64_t *samples_dev = nullptr;
+    double *std_dev = nullptr;
+    int64_t samples_cap = 0;
+    bool raw_loaded = false, sigma_set = false;
+    int last_iter = 0;
+    float last_ms = 0.f;
+    std::string err;
+};
+
+namespace {
+
+inlm_status fail(inlm_handle *h, inlm_status st, const std::string &msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return st;
+}
+
+inlm_status hip_fail(inlm_handle *h, hipError_t e, const char *where)
+{
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    return fail(h, e == hipErrorOutOfMemory ? INLM_OOM : INLM_HIP_ERROR, m);
+}
+
+#define HIP_TRY(h, expr)                                  \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+
+// Highest element index the tiled filter kernel may read in a padded volume.
+// The kernel reads whole tiles, so for shapes that are not multiples of the
+// tile it reads past the padded volume; those voxels feed only discarded
+// outputs, and the allocation covers them.
+size_t padded_alloc_elems(const inlm::Geometry &g)
+{
+    const long sp = (long)g.nxp * g.nyp;
+    const long ntz = (g.nz + 7) / 8, nty = (g.ny + 7) / 8, ntx = (g.nx + 31) / 32;
+    const long R = 3, pitch = 40;  // largest tile radius and its LDS pitch
+    const long zmax = (ntz * 8 - 1) + 2L * g.pz;
+    const long ymax = (nty * 8 - 1) + 2L * g.py;
+    const long xmax = (ntx - 1) * 32 + pitch - 1 - 2 * 0 + 2L * g.px;
+    long maxidx = zmax * sp + ymax * g.nxp + xmax + 2 * R;
+    long need = std::max<long>(maxidx + 1, (long)g.voxels_p());
+    return (size_t)need + 64;
+}
+
+inlm_status ensure_samples(inlm_handle *h, int64_t n)
+{
+    if (n <= h->samples_cap) return INLM_OK;
+    if (h->samples_dev) (void)hipFree(h->samples_dev);
+    if (h->std_dev) (void)hipFree(h->std_dev);
+    h->samples_dev = nullptr;
+    h->std_dev = nullptr;
+    h->samples_cap = 0;
+    HIP_TRY(h, hipMalloc(&h->samples_dev, sizeof(int64_t) * n));
+    HIP_TRY(h, hipMalloc(&h->std_dev, sizeof(double) * n));
+    h->samples_cap = n;
+    return INLM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void inlm_default_params(inlm_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->alpha = 0.5f;
+    p->maxiterations = 4;
+    p->nslices = 11;
+    for (int i = 0; i < 3; i++) {
+        p->radius_searchspace[i] = 10;
+        p->radius_patchspace[i] = 1;
+    }
+    p->beta = 1.f;
+    p->z_anisotropy = 1.f;
+    p->noise_mode = INLM_NOISE_ZADAPTIVE;
+    p->zwindow = 100;
+    p->n_samples = 100000;
+    p->patchsize = 15;
+    p->stds_from_mean = 1.f;
+    p->continuous_estimate = 0;
+    p->sample_seed = 0;
+    p->cpu_blockwise = 1;
+    p->precision = INLM_PREC_FP32;
+    p->weight_fn = INLM_WEIGHT_PADE;
+}
+
+int32_t inlm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *inlm_last_error(const inlm_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32_t device, inlm_handle **out)
+{
+    if (!pin || !shape_in || !out) return fail(nullptr, INLM_BAD_ARG, "null argument");
+    *out = nullptr;
+    inlm_params p = *pin;
+    int shape[3] = {shape_in[0], shape_in[1], shape_in[2]};
+    for (int i = 0; i < 3; i++) {
+        if (shape[i] < 1) return fail(nullptr, INLM_BAD_ARG, "shape must be positive");
+        if (p.radius_searchspace[i] < 0 || p.radius_patchspace[i] < 0)
+            return fail(nullptr, INLM_BAD_ARG, "radii must be non-negative");
+    }
+    if (p.nslices < 1) return fail(nullptr, INLM_BAD_ARG, "nslices must be positive");
+    // Single-slice input: the reference's main drops the z radii
+    // (main.cpp:261-265).
+    if (shape[2] == 1) {
+        p.radius_searchspace[2] = 0;
+        p.radius_patchspace[2] = 0;
+    }
+    if (p.precision != INLM_PREC_FP32)
+        return fail(nullptr, INLM_BAD_ARG, "precision form not available in this build");
+    if (p.weight_fn != INLM_WEIGHT_PADE && p.weight_fn != INLM_WEIGHT_EXP)
+        return fail(nullptr, INLM_BAD_ARG, "unknown weight function");
+
+    inlm::Geometry g = inlm::make_geometry(shape, p.radius_searchspace, p.radius_patchspace, p.nslices);
+    const int padv[3] = {g.px, g.py, g.pz};
+    for (int i = 0; i < 3; i++)
+        if (shape[i] < 2 && padv[i] > 0)
+            return fail(nullptr, INLM_BAD_ARG, "reflective padding needs at least 2 voxels on every padded axis");
+    const long sp = (long)g.nxp * g.nyp;
+    if ((double)sp * (g.pz + 8) + (double)g.nxp * 16 > 2.0e9)
+        return fail(nullptr, INLM_BAD_ARG, "slice too large for 32-bit offsets");
+
+    inlm_handle *h = new inlm_handle();
+    h->p = p;
+    h->device = device;
+    h->geo = g;
+    h->fg = inlm::FilterGeom{g.nx, g.ny, g.nz, g.nxp, g.nyp, g.nzp, g.px, g.py, g.pz};
+
+    // Tables on the padded shape (configure_device, iternlm_gpu.cu:1468-1470).
+    std::vector<inlm::Offset3> S = inlm::search_offsets(p.radius_searchspace, p.nslices);
+    std::vector<inlm::Offset3> PT = inlm::patch_offsets(p.radius_patchspace, p.nslices);
+    std::vector<float> W;
+    std::string werr;
+    if (!inlm::cpu_weight_table(p.radius_patchspace, p.nslices, p.z_anisotropy, g.nzp, p.cpu_blockwise != 0, W,
+                                werr)) {
+        delete h;
+        return fail(nullptr, INLM_BAD_ARG, werr);
+    }
+    if ((int)PT.size() > inlm::kMaxTaps) {
+        delete h;
+        return fail(nullptr, INLM_BAD_ARG, "patch has more than 128 taps");
+    }
+    if (S.empty()) {
+        delete h;
+        return fail(nullptr, INLM_BAD_ARG, "empty search space");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete h;
+        return fail(nullptr, INLM_NO_DEVICE, "no GPU visible");
+    }
+    if (device < 0 || device >= ndev) {
+        delete h;
+        return fail(nullptr, INLM_BAD_ARG, "device index out of range");
+    }
+    for (const auto &o : S) h->search.push_back({o.dx, o.dy, o.dz, (int)(o.dz * sp + (long)o.dy * g.nxp + o.dx)});
+    h->taps.n = (int)PT.size();
+    for (size_t i = 0; i < PT.size(); i++) {
+        h->taps.w[i] = W[i];
+        h->taps.lin[i] = (int)(PT[i].dz * sp + (long)PT[i].dy * g.nxp + PT[i].dx);
+        h->tap_dxyz.push_back(PT[i].dx);
+        h->tap_dxyz.push_back(PT[i].dy);
+        h->tap_dxyz.push_back(PT[i].dz);
+    }
+    const int *rp = p.radius_patchspace;
+    if (p.kernel_select == 0 && rp[0] == rp[1] && rp[1] == rp[2] && rp[0] >= 1 && rp[0] <= 3 &&
+        inlm::tiled_taps_match(rp[0], h->tap_dxyz.data(), h->taps.n))
+        h->rp_tiled = rp[0];
+
+    inlm_status st = INLM_OK;
+    auto bail = [&](inlm_status s) {
+        inlm_destroy(h);
+        return s;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipSetDevice"));
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipStreamCreate"));
+    if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipEventCreate"));
+    h->pad_elems = padded_alloc_elems(g);
+    const size_t padb = h->pad_elems * sizeof(float);
+    if ((e = hipMalloc(&h->R_pad, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(original)"));
+    if ((e = hipMalloc(&h->P_pad, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(previous)"));
+    if ((e = hipMalloc(&h->result, sizeof(float) * g.voxels())) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMalloc(result)"));
+    if ((e = hipMalloc(&h->sigma, sizeof(float) * g.nz)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMalloc(sigma)"));
+    if ((e = hipMalloc(&h->search_dev, sizeof(inlm::SearchEntry) * h->search.size())) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMalloc(search)"));
+    if ((e = hipMemsetAsync(h->R_pad, 0, padb, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->P_pad, 0, padb, h->stream)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMemset"));
+    if ((e = hipMemcpyAsync(h->search_dev, h->search.data(), sizeof(inlm::SearchEntry) * h->search.size(),
+                            hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(hip_fail(nullptr, e, "sync"));
+    (void)st;
+    *out = h;
+    return INLM_OK;
+}
+
+inlm_status inlm_destroy(inlm_handle *h)
+{
+    if (!h) return INLM_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->R_pad);
+    (void)hipFree(h->P_pad);
+    (void)hipFree(h->result);
+    (void)hipFree(h->sigma);
+    (void)hipFree(h->search_dev);
+    (void)hipFree(h->samples_dev);
+    (void)hipFree(h->std_dev);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return INLM_OK;
+}
+
+inlm_status inlm_set_sigma(inlm_handle *h, const float *sigma, int32_t nz)
+{
+    if (!h || !sigma) return fail(h, INLM_BAD_ARG, "null argument");
+    if (nz != h->geo.nz) return fail(h, INLM_BAD_ARG, "sigma list length must equal nz");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipMemcpyAsync(h->sigma, sigma, sizeof(float) * nz, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->sigma_set = true;
+    return INLM_OK;
+}
+
+inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device)
+{
+    if (!h || !raw) return fail(h, INLM_BAD_ARG, "null argument");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const size_t bytes = sizeof(float) * h->geo.voxels();
+    HIP_TRY(h, hipMemcpyAsync(h->result, raw, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              h->stream));
+    HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->R_pad, h->fg, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->raw_loaded = true;
+    h->last_iter = 0;
+    return INLM_OK;
+}
+
+inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
+{
+    if (!h) return fail(h, INLM_BAD_ARG, "null handle");
+    if (iter < 1) return fail(h, INLM_BAD_ARG, "iterations count from 1");
+    HIP_TRY(h, hipSetDevice(h->device));
+    if (iter == 1 && raw) {
+        inlm_status s = inlm_load_raw(h, raw, 0);
+        if (s != INLM_OK) return s;
+    }
+    if (!h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
+    if (!h->sigma_set) return fail(h, INLM_BAD_ARG, "sigma not set");
+    const float *P = h->R_pad;  // iteration 1: the previous iterate is the original
+    if (iter > 1) {
+        if (h->last_iter < 1) return fail(h, INLM_BAD_ARG, "iteration > 1 before iteration 1");
+        HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->P_pad, h->fg, h->stream));
+        P = h->P_pad;
+    }
+    HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
+    HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad, P, h->result, h->sigma,
+                                   h->p.beta, h->search_dev, (int)h->search.size(), h->taps, h->fg, h->stream));
+    HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+    h->last_iter = iter;
+    return INLM_OK;
+}
+
+inlm_status inlm_get_result(inlm_handle *h, float *out, int32_t is_device)
+{
+    if (!h || !out) return fail(h, INLM_BAD_ARG, "null argument");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipMemcpyAsync(out, h->result, sizeof(float) * h->geo.voxels(),
+                              is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return INLM_OK;
+}
+
+inlm_status inlm_draw_samples(const int32_t shape[3], int64_t n, int32_t patchsize, uint64_t seed, int64_t *out)
+{
+    if (!shape || (!out && n > 0)) return fail(nullptr, INLM_BAD_ARG, "null argument");
+    int sh[3] = {shape[0], shape[1], shape[2]};
+    if (!inlm::draw_samples(sh, n, patchsize, seed, out))
+        return fail(nullptr, INLM_BAD_ARG, "volume smaller than the noise patch");
+    return INLM_OK;
+}
+
+inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, double *out)
+{
+    if (!h || (!samples && n > 0) || (!out && n > 0)) return fail(h, INLM_BAD_ARG, "null argument");
+    if (which != 0 && which != 1) return fail(h, INLM_BAD_ARG, "which must be 0 (original) or 1 (result)");
+    if (which == 0 && !h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
+    if (which == 1 && h->last_iter < 1) return fail(h, INLM_BAD_ARG, "no result yet");
+    const inlm::Geometry &g = h->geo;
+    const int ps = h->p.patchsize;
+    if (ps < 1 || ps > g.nx || ps > g.ny) return fail(h, INLM_BAD_ARG, "patch size does not fit the slice");
+    const int64_t nslice = (int64_t)g.nx * g.ny;
+    for (int64_t i = 0; i < n; i++) {  // every patch must lie inside its slice
+        const int64_t idx = samples[i];
+        if (idx < 0 || idx >= g.voxels()) return fail(h, INLM_BAD_ARG, "sample index out of range");
+        const int64_t r = idx % nslice;
+        if (r % g.nx + ps > g.nx || r / g.nx + ps > g.ny) return fail(h, INLM_BAD_ARG, "sample patch leaves the slice");
+    }
+    if (n == 0) return INLM_OK;
+    HIP_TRY(h, hipSetDevice(h->device));
+    inlm_status s = ensure_samples(h, n);
+    if (s != INLM_OK) return s;
+    HIP_TRY(h, hipMemcpyAsync(h->samples_dev, samples, sizeof(int64_t) * n, hipMemcpyHostToDevice, h->stream));
+    const float *origin;
+    int64_t rowp, slp;
+    if (which == 0) {
+        const long sp = (long)g.nxp * g.nyp;
+        origin = h->R_pad + (long)g.pz * sp + (long)g.py * g.nxp + g.px;
+        rowp = g.nxp;
+        slp = sp;
+    } else {
+        origin = h->result;
+        rowp = g.nx;
+        slp = nslice;
+    }
+    HIP_TRY(h, inlm::launch_patchstd(origin, rowp, slp, h->samples_dev, n, g.nx, nslice, ps, h->std_dev, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(out, h->std_dev, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return INLM_OK;
+}
+
+inlm_status inlm_estimate_noise(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, float *sigma_out)
+{
+    if (!h || !sigma_out) return fail(h, INLM_BAD_ARG, "null argument");
+    const int nz = h->geo.nz;
+    if (h->p.noise_mode == INLM_NOISE_MANUAL) {
+        // manual mode: sigma[0] on the first estimate, sigma[1] afterwards
+        // (noiselevel.cpp:81-88).
+        const float s = h->p.sigma[which == 0 ? 0 : 1];
+        for (int i = 0; i < nz; i++) sigma_out[i] = s;
+        return INLM_OK;
+    }
+    std::vector<double> sd((size_t)std::max<int64_t>(n, 1));
+    inlm_status s = inlm_patchstd(h, which, samples, n, sd.data());
+    if (s != INLM_OK) return s;
+    const double k = h->p.stds_from_mean;
+    if (h->p.noise_mode == INLM_NOISE_ZADAPTIVE) {
+        std::vector<double> mean, stdev;
+        int shape[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
+        inlm::zwindow_clusters(samples, sd.data(), n, shape, h->p.zwindow, mean, stdev);
+        for (int i = 0; i < nz; i++) sigma_out[i] = (float)(mean[i] + k * stdev[i]);
+    } else {
+        double mean = 0, stdev = 0;
+        inlm::create2clusters(sd.data(), (int)n, 0, (int)n - 1, mean, stdev);
+        for (int i = 0; i < nz; i++) sigma_out[i] = (float)(mean + k * stdev);
+    }
+    return INLM_OK;
+}
+
+inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, float *sigma_log)
+{
+    if (!h) return fail(h, INLM_BAD_ARG, "null handle");
+    inlm_status s;
+    if (raw && (s = inlm_load_raw(h, raw, 0)) != INLM_OK) return s;
+    if (!h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
+    const int nz = h->geo.nz;
+    std::vector<int64_t> samples;
+    if (h->p.noise_mode != INLM_NOISE_MANUAL) {
+        samples.resize((size_t)h->p.n_samples);
+        int sh[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
+        if (!inlm::draw_samples(sh, h->p.n_samples, h->p.patchsize, h->p.sample_seed, samples.data()))
+            return fail(h, INLM_BAD_ARG, "volume smaller than the noise patch");
+    }
+    const int64_t ns = (int64_t)samples.size();
+    std::vector<float> s0(nz), s1(nz), tmp(nz);
+    // main.cpp:325-351: initial noise level on the original.
+    if ((s = inlm_estimate_noise(h, 0, samples.data(), ns, s0.data())) != INLM_OK) return s;
+    if ((s = inlm_set_sigma(h, s0.data(), nz)) != INLM_OK) return s;
+    if ((s = inlm_run_iteration(h, 1, nullptr)) != INLM_OK) return s;
+    if (sigma_log) std::memcpy(sigma_log, s0.data(), sizeof(float) * nz);
+    auto reestimate = [&]() -> inlm_status {
+        // main.cpp:420-448: re-estimate on the current result with the same
+        // samples, then sigma = alpha*s0 + (1-alpha)*s1 in float.
+        inlm_status r = inlm_estimate_noise(h, 1, samples.data(), ns, tmp.data());
+        if (r != INLM_OK) return r;
+        const float a = h->p.alpha;
+        for (int i = 0; i < nz; i++) s1[i] = a * s0[i] + (1.f - a) * tmp[i];
+        return inlm_set_sigma(h, s1.data(), nz);
+    };
+    if (h->p.maxiterations > 1 && (s = reestimate()) != INLM_OK) return s;
+    for (int iter = 2; iter <= h->p.maxiterations; iter++) {
+        if ((s = inlm_run_iteration(h, iter, nullptr)) != INLM_OK) return s;
+        if (sigma_log) std::memcpy(sigma_log + (size_t)(iter - 1) * nz, s1.data(), sizeof(float) * nz);
+        if (h->p.continuous_estimate && h->p.maxiterations > iter + 1 && (s = reestimate()) != INLM_OK) return s;
+    }
+    if (out) return inlm_get_result(h, out, 0);
+    return INLM_OK;
+}
+
+inlm_status inlm_tables(const inlm_params *p, const int32_t shape[3], int32_t cap, int32_t *n_search,
+                        int32_t *search_dxyz, int32_t *n_patch, int32_t *patch_dxyz, float *weights)
+{
+    if (!p || !shape || !n_search || !n_patch) return fail(nullptr, INLM_BAD_ARG, "null argument");
+    int sh[3] = {shape[0], shape[1], shape[2]};
+    inlm::Geometry g = inlm::make_geometry(sh, p->radius_searchspace, p->radius_patchspace, p->nslices);
+    std::vector<inlm::Offset3> S = inlm::search_offsets(p->radius_searchspace, p->nslices);
+    std::vector<inlm::Offset3> PT = inlm::patch_offsets(p->radius_patchspace, p->nslices);
+    std::vector<float> W;
+    std::string err;
+    if (!inlm::cpu_weight_table(p->radius_patchspace, p->nslices, p->z_anisotropy, g.nzp, p->cpu_blockwise != 0, W,
+                                err))
+        return fail(nullptr, INLM_BAD_ARG, err);
+    *n_search = (int32_t)S.size();
+    *n_patch = (int32_t)PT.size();
+    for (int i = 0; i < (int)S.size() && i < cap && search_dxyz; i++) {
+        search_dxyz[3 * i] = S[i].dx;
+        search_dxyz[3 * i + 1] = S[i].dy;
+        search_dxyz[3 * i + 2] = S[i].dz;
+    }
+    for (int i = 0; i < (int)PT.size() && i < cap; i++) {
+        if (patch_dxyz) {
+            patch_dxyz[3 * i] = PT[i].dx;
+            patch_dxyz[3 * i + 1] = PT[i].dy;
+            patch_dxyz[3 * i + 2] = PT[i].dz;
+        }
+        if (weights) weights[i] = W[i];
+    }
+    return INLM_OK;
+}
+
+float inlm_last_filter_ms(const inlm_handle *h) { return h ? h->last_ms : 0.f; }
+
+inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n_patch)
+{
+    if (!h) return INLM_BAD_ARG;
+    if (n_search) *n_search = (int32_t)h->search.size();
+    if (n_patch) *n_patch = h->taps.n;
+    return INLM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,494 @@
+// iterNLM filter, padding and noise kernels for gfx950 (MI355X).
+//
+// The filter pass computes, per voxel x of slice z (sigma = sigma[z],
+// m = -1/(sigma^2*beta)):
+//   d_s(x) = sum_p W_p * (P[x+s+p] - P[x+p])^2       (taps p in reference order)
+//   w_s    = pade(d_s*m) if d_s*m > -3.56648 else 0   (or expf)
+//   num += w_s*R[x+s]; den += w_s; wmax = max(wmax, w_s)
+//   out    = (num + wmax*R[x]) / (den + wmax)          (R[x] when wmax == 0)
+// which restates Denoiser/iternlm_cpu.cpp:758-843 and the specialised
+// Denoiser/iternlm_cpu_kernels.cpp:7-1108 of the reference.
+//
+// The tiled kernels are offset-major: a workgroup owns a 32x8x8 output tile;
+// for each search offset s it stages D_s(y) = (P(y+s) - P(y))^2 over the tile
+// plus the patch halo in LDS (double buffered, one barrier per offset) and
+// every thread sums the stencil for a run of 8 voxels along x from LDS
+// windows.  D_s(x+p) is the very float the reference computes as tmp*tmp for
+// tap p, and every voxel adds its taps in the reference order with separate
+// multiply and add (this file is compiled with -ffp-contract=off), so the
+// result is bit-identical to the reference CPU path.
+#include <hip/hip_runtime.h>
+
+#include "inlm_kernels.hpp"
+
+namespace inlm {
+namespace {
+
+constexpr int NT = 256;  // threads per workgroup (4 wavefronts)
+constexpr int TX = 32, TY = 8, TZ = 8, VX = 8;
+constexpr float kPadeCut = -3.56648f;
+
+// ---------------------------------------------------------------------------
+// Compile-time patch rows of an isotropic ellipsoid of radius R, in the
+// reference scan order z -> y -> x (iternlm_prepare.cpp:152-203).  Each row
+// (dz,dy) holds the taps dx = -j..j; tap0 is the reference index of its first
+// tap (index 0 is the centre, which leads the list).
+// ---------------------------------------------------------------------------
+struct Row {
+    int dz, dy, j, tap0;
+};
+template <int R>
+struct Rows {
+    Row r[4 * R * R + 4 * R + 1];
+    int n, ntaps;
+    static constexpr bool inside(int x, int y, int z)
+    {
+        const float rr = (float)R;
+        const float fx = (float)x / rr, fy = (float)y / rr, fz = (float)z / rr;
+        float e = fx * fx;
+        e = e + fy * fy;
+        e = e + fz * fz;
+        return e <= 1.f;
+    }
+    constexpr Rows() : r(), n(0), ntaps(0)
+    {
+        int tap = 1;
+        for (int z = -R; z <= R; ++z)
+            for (int y = -R; y <= R; ++y) {
+                int j = -1;
+                for (int x = 0; x <= R; ++x)
+                    if (inside(x, y, z)) j = x;
+                if (j < 0) continue;
+                r[n] = Row{z, y, j, tap};
+                n++;
+                tap += 2 * j + 1 - ((z == 0 && y == 0) ? 1 : 0);
+            }
+        ntaps = tap;
+    }
+};
+
+template <int R>
+struct Tile {
+    static constexpr int RX = TX + 2 * R, RY = TY + 2 * R, RZ = TZ + 2 * R;
+    static constexpr int PITCH = (RX + 3) & ~3;  // LDS row pitch in floats
+    static constexpr int CPR = PITCH / 4;        // 4-float chunks per row
+    static constexpr int NCHUNK = RZ * RY * CPR;
+    static constexpr int CPT = (NCHUNK + NT - 1) / NT;  // chunks staged per thread
+    static constexpr int LDS = RZ * RY * PITCH;         // floats per buffer
+    static constexpr int WIN = ((VX + 2 * R) + 3) & ~3; // floats read per row window
+};
+
+struct TiledArgs {
+    const float *R;
+    const float *P;
+    float *out;
+    const float *sigma;
+    const SearchEntry *search;
+    int n_search;
+    float beta;
+    FilterGeom g;
+    int ntx, nty, ntz;
+    float w[kMaxTaps];
+};
+
+__device__ __forceinline__ float pade_cut(float a)
+{
+    // (120 + 60a + 12a^2 + a^3) / (120 - 60a + 12a^2 - a^3), evaluated left to
+    // right as Denoiser/iternlm_cpu.h:125 writes it; IEEE division.
+    const float t60 = 60.f * a;
+    const float t12 = (12.f * a) * a;
+    const float t3 = (a * a) * a;
+    float num = 120.f + t60;
+    num = num + t12;
+    num = num + t3;
+    float den = 120.f - t60;
+    den = den + t12;
+    den = den - t3;
+    const float q = num / den;
+    return (a > kPadeCut) ? q : 0.0f;
+}
+
+template <int WFN>
+__device__ __forceinline__ float weight_of(float a)
+{
+    if constexpr (WFN == 0) return pade_cut(a);
+    else return expf(a);
+}
+
+// Workgroup id -> tile, so that the tiles one XCD runs are contiguous (the
+// hardware deals workgroups to the 8 XCDs round-robin, and neighbouring tiles
+// share their halo in that XCD's L2).
+__device__ __forceinline__ int xcd_swizzle(int wg, int nwg)
+{
+    if ((nwg & 7) != 0) return wg;
+    return (wg & 7) * (nwg >> 3) + (wg >> 3);
+}
+
+template <int R, int WFN>
+__global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
+{
+    using T = Tile<R>;
+    constexpr Rows<R> rows{};
+    __shared__ float lds[2][T::LDS];
+
+    const int t = threadIdx.x;
+    const int wg = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int bx = wg % a.ntx;
+    const int rem = wg / a.ntx;
+    const int by = rem % a.nty;
+    const int bz = rem / a.nty;
+    const int X0 = bx * TX, Y0 = by * TY, Z0 = bz * TZ;
+    const FilterGeom g = a.g;
+    const long sp = (long)g.nxp * g.nyp;
+
+    // Staging: region origin (padded coordinates) is the tile origin minus R.
+    const float *Pbase = a.P + ((long)(Z0 + g.pz - R) * sp + (long)(Y0 + g.py - R) * g.nxp + (X0 + g.px - R));
+    int coff[T::CPT];
+    float p0[T::CPT][4];
+#pragma unroll
+    for (int c = 0; c < T::CPT; c++) {
+        int ci = t + c * NT;
+        if (ci >= T::NCHUNK) ci = 0;  // duplicate of chunk 0, never written
+        const int cx = ci % T::CPR, row = ci / T::CPR;
+        const int ry = row % T::RY, rz = row / T::RY;
+        coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
+#pragma unroll
+        for (int e = 0; e < 4; e++) p0[c][e] = Pbase[coff[c] + e];
+    }
+
+    // This thread's 8-voxel run.
+    const int xr = t & 3, ty = (t >> 2) & 7, tz = t >> 5;
+    const int x0 = X0 + 8 * xr, y = Y0 + ty, z = Z0 + tz;
+    const long vox = (long)(z + g.pz) * sp + (long)(y + g.py) * g.nxp + (x0 + g.px);
+    const float *Rv = a.R + vox;
+    const int zc = z < g.nz ? z : g.nz - 1;
+    const float sig = a.sigma[zc];
+    const float m = -1.f / (sig * sig * a.beta);
+
+    float fv[VX], fw[VX], wmax[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) fv[v] = fw[v] = wmax[v] = 0.f;
+
+    const int ldsrow = ((tz + R) * T::RY + (ty + R)) * T::PITCH + 8 * xr;
+    const int nS = a.n_search;
+
+    auto stage = [&](float (&pn)[T::CPT][4], float *buf) {
+#pragma unroll
+        for (int c = 0; c < T::CPT; c++) {
+            const int ci = t + c * NT;
+            if (T::NCHUNK % NT == 0 || ci < T::NCHUNK) {
+                float4 dv;
+                float q;
+                q = pn[c][0] - p0[c][0]; dv.x = q * q;
+                q = pn[c][1] - p0[c][1]; dv.y = q * q;
+                q = pn[c][2] - p0[c][2]; dv.z = q * q;
+                q = pn[c][3] - p0[c][3]; dv.w = q * q;
+                *reinterpret_cast<float4 *>(buf + 4 * ci) = dv;
+            }
+        }
+    };
+    auto load_chunks = [&](float (&pn)[T::CPT][4], int lin) {
+        const float *base = Pbase + lin;
+#pragma unroll
+        for (int c = 0; c < T::CPT; c++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) pn[c][e] = base[coff[c] + e];
+    };
+
+    {
+        float pn[T::CPT][4];
+        load_chunks(pn, a.search[0].lin);
+        stage(pn, lds[0]);
+    }
+    __syncthreads();
+
+    for (int k = 0; k < nS; k++) {
+        const int lin = a.search[k].lin;
+        const bool more = (k + 1) < nS;
+        float pn[T::CPT][4];
+        if (more) load_chunks(pn, a.search[k + 1].lin);
+        float rs[VX];
+#pragma unroll
+        for (int v = 0; v < VX; v++) rs[v] = Rv[lin + v];
+
+        const float *L = lds[k & 1] + ldsrow;
+        float d[VX];
+        {
+            float win[T::WIN];
+#pragma unroll
+            for (int i = 0; i < T::WIN; i++) win[i] = L[i];
+#pragma unroll
+            for (int v = 0; v < VX; v++) d[v] = win[R + v] * a.w[0];
+        }
+#pragma unroll
+        for (int ri = 0; ri < rows.n; ri++) {
+            const Row rw = rows.r[ri];
+            const float *Lr = L + (rw.dz * T::RY + rw.dy) * T::PITCH;
+            float win[T::WIN];
+#pragma unroll
+            for (int i = 0; i < T::WIN; i++) win[i] = Lr[i];
+            int tap = rw.tap0;
+#pragma unroll
+            for (int dx = -rw.j; dx <= rw.j; dx++) {
+                if (rw.dz == 0 && rw.dy == 0 && dx == 0) continue;
+                const float wt = a.w[tap];
+#pragma unroll
+                for (int v = 0; v < VX; v++) d[v] = d[v] + win[R + v + dx] * wt;
+                tap++;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            const float wt = weight_of<WFN>(d[v] * m);
+            fv[v] = fv[v] + wt * rs[v];
+            fw[v] = fw[v] + wt;
+            wmax[v] = (wt > wmax[v]) ? wt : wmax[v];
+        }
+        if (more) stage(pn, lds[(k + 1) & 1]);
+        __syncthreads();
+    }
+
+    if (y < g.ny && z < g.nz) {
+        float *o = a.out + ((long)z * g.ny + y) * g.nx + x0;
+#pragma unroll
+        for (int v = 0; v < VX; v++) {
+            if (x0 + v < g.nx) {
+                const float r0 = Rv[v];
+                float res;
+                if (wmax[v] > 0.0f) {
+                    const float num = fv[v] + wmax[v] * r0;
+                    const float den = fw[v] + wmax[v];
+                    res = num / den;
+                } else
+                    res = r0;
+                o[v] = res;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-voxel kernel for any tap table (non-isotropic patches, anisotropic
+// weights): one thread per voxel, gathers straight from global memory, the
+// same operation sequence as the tiled kernel.
+// ---------------------------------------------------------------------------
+struct VoxelArgs {
+    const float *R;
+    const float *P;
+    float *out;
+    const float *sigma;
+    const SearchEntry *search;
+    int n_search;
+    float beta;
+    FilterGeom g;
+    TapTable taps;
+};
+
+template <int WFN>
+__global__ __launch_bounds__(NT) void filter_pervoxel_fp32(VoxelArgs a)
+{
+    const FilterGeom g = a.g;
+    const long nvox = (long)g.nx * g.ny * g.nz;
+    const long idx = (long)blockIdx.x * NT + threadIdx.x;
+    if (idx >= nvox) return;
+    const int x = (int)(idx % g.nx);
+    const int y = (int)((idx / g.nx) % g.ny);
+    const int z = (int)(idx / ((long)g.nx * g.ny));
+    const long sp = (long)g.nxp * g.nyp;
+    const long i0 = (long)(z + g.pz) * sp + (long)(y + g.py) * g.nxp + (x + g.px);
+    const float sig = a.sigma[z];
+    const float m = -1.f / (sig * sig * a.beta);
+    float fv = 0.f, fw = 0.f, wmax = 0.f;
+    for (int k = 0; k < a.n_search; k++) {
+        const long i1 = i0 + a.search[k].lin;
+        float d = 0.f;
+        for (int p = 0; p < a.taps.n; p++) {
+            const float q = a.P[i1 + a.taps.lin[p]] - a.P[i0 + a.taps.lin[p]];
+            d = d + (q * q) * a.taps.w[p];
+        }
+        const float wt = weight_of<WFN>(d * m);
+        fv = fv + wt * a.R[i1];
+        fw = fw + wt;
+        wmax = (wt > wmax) ? wt : wmax;
+    }
+    const float r0 = a.R[i0];
+    float res = r0;
+    if (wmax > 0.0f) {
+        const float num = fv + wmax * r0;
+        const float den = fw + wmax;
+        res = num / den;
+    }
+    a.out[idx] = res;
+}
+
+// ---------------------------------------------------------------------------
+// Reflective padding (iternlm_prepare.cpp:6-48): mirror without repeating the
+// edge, iterated until in range.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void pad_reflect_kernel(const float *__restrict__ in, float *__restrict__ out,
+                                                         FilterGeom g)
+{
+    const long n = (long)g.nxp * g.nyp * g.nzp;
+    const long idx = (long)blockIdx.x * NT + threadIdx.x;
+    if (idx >= n) return;
+    const int x1 = (int)(idx % g.nxp);
+    const int y1 = (int)((idx / g.nxp) % g.nyp);
+    const int z1 = (int)(idx / ((long)g.nxp * g.nyp));
+    int x = x1 - g.px, y = y1 - g.py, z = z1 - g.pz;
+    while (x < 0 || x >= g.nx) x = (x < 0) ? -x : 2 * g.nx - x - 2;
+    while (y < 0 || y >= g.ny) y = (y < 0) ? -y : 2 * g.ny - y - 2;
+    while (z < 0 || z >= g.nz) z = (z < 0) ? -z : 2 * g.nz - z - 2;
+    out[idx] = in[((long)z * g.ny + y) * g.nx + x];
+}
+
+// ---------------------------------------------------------------------------
+// Noise: one wavefront per sample computes the population standard deviation
+// of its patchsize x patchsize in-slice patch (noiselevel.cpp:235-283) with
+// FP64 two-pass accumulation and a wavefront butterfly reduction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(NT) void patchstd_kernel(const float *__restrict__ origin, long row_pitch,
+                                                      long slice_pitch, const int64_t *__restrict__ samples,
+                                                      long n, int nx, long nslice, int ps, double *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const long s = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (s >= n) return;  // whole wavefront leaves together
+    const long idx = samples[s];
+    const int z0 = (int)(idx / nslice);
+    const int y0 = (int)((idx - z0 * nslice) / nx);
+    const int x0 = (int)(idx - z0 * nslice - (long)y0 * nx);
+    const float *base = origin + (long)z0 * slice_pitch + (long)y0 * row_pitch + x0;
+    const int plen = ps * ps;
+    double sum = 0.0;
+    for (int i = lane; i < plen; i += 64) sum += (double)base[(long)(i / ps) * row_pitch + (i % ps)];
+    const double mean = wave_sum(sum) / (double)plen;
+    double sq = 0.0;
+    for (int i = lane; i < plen; i += 64) {
+        const double dv = (double)base[(long)(i / ps) * row_pitch + (i % ps)] - mean;
+        sq += dv * dv;
+    }
+    const double var = wave_sum(sq) / (double)plen;
+    if (lane == 0) out[s] = sqrt(var);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <int R>
+static bool taps_match_r(const int *dxyz, int n)
+{
+    constexpr Rows<R> rows{};
+    if (n != rows.ntaps) return false;
+    if (dxyz[0] != 0 || dxyz[1] != 0 || dxyz[2] != 0) return false;
+    for (int ri = 0; ri < rows.n; ri++) {
+        const Row rw = rows.r[ri];
+        int tap = rw.tap0;
+        for (int dx = -rw.j; dx <= rw.j; dx++) {
+            if (rw.dz == 0 && rw.dy == 0 && dx == 0) continue;
+            if (dxyz[3 * tap] != dx || dxyz[3 * tap + 1] != rw.dy || dxyz[3 * tap + 2] != rw.dz) return false;
+            tap++;
+        }
+    }
+    return true;
+}
+
+bool tiled_taps_match(int rp, const int *dxyz, int n)
+{
+    switch (rp) {
+    case 1: return taps_match_r<1>(dxyz, n);
+    case 2: return taps_match_r<2>(dxyz, n);
+    case 3: return taps_match_r<3>(dxyz, n);
+    default: return false;
+    }
+}
+
+hipError_t launch_pad_reflect(const float *in, float *out, const FilterGeom &g, hipStream_t st)
+{
+    const long n = (long)g.nxp * g.nyp * g.nzp;
+    const unsigned blocks = (unsigned)((n + NT - 1) / NT);
+    hipLaunchKernelGGL(pad_reflect_kernel, dim3(blocks), dim3(NT), 0, st, in, out, g);
+    return hipGetLastError();
+}
+
+template <int R, int WFN>
+static hipError_t launch_tiled(const float *Rv, const float *P, float *out, const float *sigma, float beta,
+                               const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
+                               hipStream_t st)
+{
+    TiledArgs a;
+    a.R = Rv;
+    a.P = P;
+    a.out = out;
+    a.sigma = sigma;
+    a.search = search;
+    a.n_search = n_search;
+    a.beta = beta;
+    a.g = g;
+    a.ntx = (g.nx + TX - 1) / TX;
+    a.nty = (g.ny + TY - 1) / TY;
+    a.ntz = (g.nz + TZ - 1) / TZ;
+    for (int i = 0; i < kMaxTaps; i++) a.w[i] = i < taps.n ? taps.w[i] : 0.f;
+    const long nwg = (long)a.ntx * a.nty * a.ntz;
+    hipLaunchKernelGGL((filter_tiled_fp32<R, WFN>), dim3((unsigned)nwg), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int WFN>
+static hipError_t launch_pervoxel(const float *Rv, const float *P, float *out, const float *sigma, float beta,
+                                  const SearchEntry *search, int n_search, const TapTable &taps,
+                                  const FilterGeom &g, hipStream_t st)
+{
+    VoxelArgs a;
+    a.R = Rv;
+    a.P = P;
+    a.out = out;
+    a.sigma = sigma;
+    a.search = search;
+    a.n_search = n_search;
+    a.beta = beta;
+    a.g = g;
+    a.taps = taps;
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL((filter_pervoxel_fp32<WFN>), dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const float *P, float *out,
+                         const float *sigma, float beta, const SearchEntry *search, int n_search,
+                         const TapTable &taps, const FilterGeom &g, hipStream_t st)
+{
+    if (form != FORM_FP32) return hipErrorInvalidValue;
+#define INLM_TILED(RR)                                                                                   \
+    return weight_fn == 0 ? launch_tiled<RR, 0>(R, P, out, sigma, beta, search, n_search, taps, g, st)   \
+                          : launch_tiled<RR, 1>(R, P, out, sigma, beta, search, n_search, taps, g, st)
+    switch (rp) {
+    case 1: INLM_TILED(1);
+    case 2: INLM_TILED(2);
+    case 3: INLM_TILED(3);
+    default: break;
+    }
+#undef INLM_TILED
+    return weight_fn == 0 ? launch_pervoxel<0>(R, P, out, sigma, beta, search, n_search, taps, g, st)
+                          : launch_pervoxel<1>(R, P, out, sigma, beta, search, n_search, taps, g, st);
+}
+
+hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice_pitch, const int64_t *samples,
+                           int64_t n, int nx, int64_t nslice, int patchsize, double *out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + (NT / 64) - 1) / (NT / 64));
+    hipLaunchKernelGGL(patchstd_kernel, dim3(blocks), dim3(NT), 0, st, origin, (long)row_pitch, (long)slice_pitch,
+                       samples, (long)n, nx, (long)nslice, patchsize, out);
+    return hipGetLastError();
+}
+
+}  // namespace inlm

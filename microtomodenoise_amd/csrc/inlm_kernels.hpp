@@ -1,0 +1,55 @@
+// Launch wrappers of the iterNLM HIP kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace inlm {
+
+// Offset entry of the search table: shift in voxels plus the linear shift in
+// the padded layout.
+struct SearchEntry {
+    int dx, dy, dz;
+    int lin;  // dz*nxp*nyp + dy*nxp + dx (fits in 32 bits for every supported shape)
+};
+
+struct FilterGeom {
+    int nx, ny, nz;
+    int nxp, nyp, nzp;
+    int px, py, pz;
+};
+
+enum FilterForm { FORM_FP32 = 0, FORM_FP32_FAST = 1, FORM_BF16 = 2, FORM_FP16 = 3 };
+
+constexpr int kMaxTaps = 128;
+
+struct TapTable {
+    float w[kMaxTaps];
+    int lin[kMaxTaps];  // linear patch shifts in the padded layout
+    int n;
+};
+
+// Reflect-pad an unpadded float volume into a padded float volume
+// (mirror without repeating the edge, iternlm_prepare.cpp:6-48).
+hipError_t launch_pad_reflect(const float *in, float *out, const FilterGeom &g, hipStream_t st);
+
+// One filter pass.  R/P are padded volumes; out is unpadded; sigma has nz
+// entries.  Returns hipErrorInvalidValue when no kernel covers the patch.
+// rp is the (isotropic) patch radius served by a tiled kernel, or 0 to force
+// the per-voxel kernel, which runs any tap table.
+hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const float *P, float *out,
+                         const float *sigma, float beta, const SearchEntry *search, int n_search,
+                         const TapTable &taps, const FilterGeom &g, hipStream_t st);
+
+// Whether the tiled kernel for isotropic radius rp uses exactly the given tap
+// order (dx,dy,dz per tap, centre first).
+bool tiled_taps_match(int rp, const int *dxyz, int n);
+
+// Patch standard deviation per sample (one wavefront per sample).  Sample
+// indices are linear in the unpadded volume (nx, nslice = nx*ny); the volume
+// itself is addressed through `origin` (voxel 0,0,0) with the given row and
+// slice pitches, so the padded original can be read in place.
+hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice_pitch, const int64_t *samples,
+                           int64_t n, int nx, int64_t nslice, int patchsize, double *out, hipStream_t st);
+
+}  // namespace inlm

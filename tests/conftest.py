@@ -1,0 +1,32 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+def load_npz(name):
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def filter_fixtures():
+    return sorted(f for f in os.listdir(GOLDEN) if f.startswith("filter_") and f.endswith(".npz"))
+
+
+@pytest.fixture(scope="session")
+def gpu_available():
+    from microtomodenoise_amd import _capi
+    n = _capi.lib().inlm_device_count()
+    assert n >= 1, "GPU tests need a visible GPU: the HIP library reports none"
+    return n
