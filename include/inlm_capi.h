@@ -171,6 +171,14 @@ INLM_API inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, 
 INLM_API inlm_status inlm_tables(const inlm_params *p, const int32_t shape[3], int32_t cap, int32_t *n_search,
                                  int32_t *search_dxyz, int32_t *n_patch, int32_t *patch_dxyz, float *weights);
 
+/* Self-test of the division inside the FP32 parity kernel's Pade weight: on
+ * `device`, evaluates the weight for every float exponent from -0 down to the
+ * cutoff -3.56648 (about 1.08e9 values) with the kernel's division and with
+ * IEEE division and counts the results that differ.  It has no counterpart
+ * in the reference; it backs the bit-exactness claim of INLM_PREC_FP32. */
+INLM_API inlm_status inlm_selftest_division(int32_t device, uint64_t *mismatches, uint64_t *checked,
+                                            uint32_t *example);
+
 /* Kernel time of the last filter pass in milliseconds (HIP events recorded
  * on the handle's stream around the filter launch). */
 INLM_API float inlm_last_filter_ms(const inlm_handle *h);

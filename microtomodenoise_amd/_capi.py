@@ -25,8 +25,8 @@ STATUS_NAMES = {0: "OK", -1: "NO_DEVICE", -2: "HIP_ERROR", -3: "BAD_ARG", -4: "O
 EXPORTS = [
     "inlm_default_params", "inlm_device_count", "inlm_create", "inlm_destroy", "inlm_last_error",
     "inlm_set_sigma", "inlm_load_raw", "inlm_run_iteration", "inlm_get_result", "inlm_draw_samples",
-    "inlm_patchstd", "inlm_estimate_noise", "inlm_denoise", "inlm_tables", "inlm_last_filter_ms",
-    "inlm_table_sizes",
+    "inlm_patchstd", "inlm_estimate_noise", "inlm_denoise", "inlm_tables", "inlm_selftest_division",
+    "inlm_last_filter_ms", "inlm_table_sizes",
 ]
 
 
@@ -93,6 +93,8 @@ def lib():
         L.inlm_last_filter_ms.restype = C.c_float
         L.inlm_table_sizes.argtypes = [H, i32p, i32p]
         L.inlm_tables.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, i32p, i32p, i32p, i32p, f32p]
+        L.inlm_selftest_division.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint32)]
         _LIB = L
     return _LIB
 

@@ -202,13 +202,16 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
     for (size_t i = 0; i < PT.size(); i++) {
         h->taps.w[i] = W[i];
         h->taps.lin[i] = (int)(PT[i].dz * sp + (long)PT[i].dy * g.nxp + PT[i].dx);
+        h->taps.dxyz[3 * i] = PT[i].dx;
+        h->taps.dxyz[3 * i + 1] = PT[i].dy;
+        h->taps.dxyz[3 * i + 2] = PT[i].dz;
         h->tap_dxyz.push_back(PT[i].dx);
         h->tap_dxyz.push_back(PT[i].dy);
         h->tap_dxyz.push_back(PT[i].dz);
     }
     const int *rp = p.radius_patchspace;
     if (p.kernel_select == 0 && rp[0] == rp[1] && rp[1] == rp[2] && rp[0] >= 1 && rp[0] <= 3 &&
-        inlm::tiled_taps_match(rp[0], h->tap_dxyz.data(), h->taps.n))
+        inlm::tiled_taps_match(rp[0], h->tap_dxyz.data(), h->taps.w, h->taps.n))
         h->rp_tiled = rp[0];
 
     inlm_status st = INLM_OK;
@@ -469,6 +472,33 @@ inlm_status inlm_tables(const inlm_params *p, const int32_t shape[3], int32_t ca
         }
         if (weights) weights[i] = W[i];
     }
+    return INLM_OK;
+}
+
+inlm_status inlm_selftest_division(int32_t device, uint64_t *mismatches, uint64_t *checked, uint32_t *example)
+{
+    if (!mismatches || !checked) return fail(nullptr, INLM_BAD_ARG, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, INLM_NO_DEVICE, "no GPU visible");
+    if (device < 0 || device >= ndev) return fail(nullptr, INLM_BAD_ARG, "device index out of range");
+    HIP_TRY(nullptr, hipSetDevice(device));
+    unsigned long long *cnt = nullptr;
+    HIP_TRY(nullptr, hipMalloc(&cnt, 16));
+    // bit patterns of -0 ... the cutoff -3.56648f
+    const float cut = -3.56648f;
+    uint32_t cb;
+    std::memcpy(&cb, &cut, 4);
+    const unsigned count = (cb - 0x80000000u) + 1u;
+    hipError_t e = hipMemset(cnt, 0, 16);
+    if (e == hipSuccess) e = inlm::launch_selftest_div(0u, count, cnt, reinterpret_cast<unsigned *>(cnt + 1), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    unsigned long long host[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpy(host, cnt, 16, hipMemcpyDeviceToHost);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "division self-test");
+    *mismatches = host[0];
+    *checked = count;
+    if (example) *example = (uint32_t)host[1];
     return INLM_OK;
 }
 

@@ -19,28 +19,36 @@
 // result is bit-identical to the reference CPU path.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "inlm_kernels.hpp"
 
 namespace inlm {
 namespace {
 
-constexpr int NT = 256;  // threads per workgroup (4 wavefronts)
-constexpr int TX = 32, TY = 8, TZ = 8, VX = 8;
+constexpr int NT = 256;   // threads per workgroup of the helper kernels
+constexpr int NTF = 512;  // threads per workgroup of the tiled filter (8 wavefronts)
+constexpr int TX = 32, TY = 8, TZ = 8, VX = 4;  // tile; each thread sums a 4-voxel x-run
 constexpr float kPadeCut = -3.56648f;
 
 // ---------------------------------------------------------------------------
 // Compile-time patch rows of an isotropic ellipsoid of radius R, in the
 // reference scan order z -> y -> x (iternlm_prepare.cpp:152-203).  Each row
 // (dz,dy) holds the taps dx = -j..j; tap0 is the reference index of its first
-// tap (index 0 is the centre, which leads the list).
+// tap (index 0 is the centre, which leads the list).  Every tap also carries a
+// weight class: the centre is class 0, other taps are grouped by
+// (dx^2+dy^2, dz^2), which fixes 1/(2d+1)^2 for any z anisotropy, so the
+// kernel needs one scalar weight per class instead of one per tap.
 // ---------------------------------------------------------------------------
 struct Row {
     int dz, dy, j, tap0;
 };
+constexpr int kMaxClasses = 48;
 template <int R>
 struct Rows {
     Row r[4 * R * R + 4 * R + 1];
-    int n, ntaps;
+    int n, ntaps, nclass;
+    int key_rho[kMaxClasses], key_z[kMaxClasses];
     static constexpr bool inside(int x, int y, int z)
     {
         const float rr = (float)R;
@@ -50,7 +58,14 @@ struct Rows {
         e = e + fz * fz;
         return e <= 1.f;
     }
-    constexpr Rows() : r(), n(0), ntaps(0)
+    constexpr int class_of(int dx, int dy, int dz) const
+    {
+        if (dx == 0 && dy == 0 && dz == 0) return 0;
+        for (int c = 1; c < nclass; c++)
+            if (key_rho[c] == dx * dx + dy * dy && key_z[c] == dz * dz) return c;
+        return -1;
+    }
+    constexpr Rows() : r(), n(0), ntaps(0), nclass(1), key_rho(), key_z()
     {
         int tap = 1;
         for (int z = -R; z <= R; ++z)
@@ -62,10 +77,39 @@ struct Rows {
                 r[n] = Row{z, y, j, tap};
                 n++;
                 tap += 2 * j + 1 - ((z == 0 && y == 0) ? 1 : 0);
+                for (int x = -j; x <= j; ++x) {
+                    if (x == 0 && y == 0 && z == 0) continue;
+                    if (class_of(x, y, z) < 0) {
+                        key_rho[nclass] = x * x + y * y;
+                        key_z[nclass] = z * z;
+                        nclass++;
+                    }
+                }
             }
         ntaps = tap;
     }
 };
+template <int R>
+struct RowsOf {
+    static constexpr Rows<R> value{};
+    static_assert(value.nclass <= kMaxClasses, "too many weight classes");
+};
+
+// Compiler fence that also pins the running sums: the row's arithmetic must
+// complete before it and no memory access moves across it.
+__device__ __forceinline__ void pin(float (&d)[4])
+{
+    asm volatile("" : "+v"(d[0]), "+v"(d[1]), "+v"(d[2]), "+v"(d[3])::"memory");
+}
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
 
 template <int R>
 struct Tile {
@@ -73,7 +117,7 @@ struct Tile {
     static constexpr int PITCH = (RX + 3) & ~3;  // LDS row pitch in floats
     static constexpr int CPR = PITCH / 4;        // 4-float chunks per row
     static constexpr int NCHUNK = RZ * RY * CPR;
-    static constexpr int CPT = (NCHUNK + NT - 1) / NT;  // chunks staged per thread
+    static constexpr int CPT = (NCHUNK + NTF - 1) / NTF;  // chunks staged per thread
     static constexpr int LDS = RZ * RY * PITCH;         // floats per buffer
     static constexpr int WIN = ((VX + 2 * R) + 3) & ~3; // floats read per row window
 };
@@ -88,13 +132,34 @@ struct TiledArgs {
     float beta;
     FilterGeom g;
     int ntx, nty, ntz;
-    float w[kMaxTaps];
+    float wc[kMaxClasses];  // weight per class (class 0 = centre)
 };
 
+// Correctly rounded n/d for the operands the Pade weight produces
+// (n in [0, 120], d in [120, 600]: no overflow, underflow or special value):
+// a reciprocal refined by one Newton step, then two remainder corrections
+// with exact fused remainders.  This is the IEEE division sequence without the
+// range scaling and special-value fixup that these operands never need;
+// filter_selftest_div checks it against IEEE division on every float in the
+// weight's domain.
+__device__ __forceinline__ float div_pade(float n, float d)
+{
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = n * y;
+    float r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(r, y, q);
+}
+
+// (120 + 60a + 12a^2 + a^3) / (120 - 60a + 12a^2 - a^3), evaluated left to
+// right with every product and sum rounded, as Denoiser/iternlm_cpu.h:125
+// writes it, and zero at or below the cutoff of iternlm_cpu.h:126.
+template <bool IEEE_DIV>
 __device__ __forceinline__ float pade_cut(float a)
 {
-    // (120 + 60a + 12a^2 + a^3) / (120 - 60a + 12a^2 - a^3), evaluated left to
-    // right as Denoiser/iternlm_cpu.h:125 writes it; IEEE division.
     const float t60 = 60.f * a;
     const float t12 = (12.f * a) * a;
     const float t3 = (a * a) * a;
@@ -104,14 +169,14 @@ __device__ __forceinline__ float pade_cut(float a)
     float den = 120.f - t60;
     den = den + t12;
     den = den - t3;
-    const float q = num / den;
+    const float q = IEEE_DIV ? num / den : div_pade(num, den);
     return (a > kPadeCut) ? q : 0.0f;
 }
 
 template <int WFN>
 __device__ __forceinline__ float weight_of(float a)
 {
-    if constexpr (WFN == 0) return pade_cut(a);
+    if constexpr (WFN == 0) return pade_cut<false>(a);
     else return expf(a);
 }
 
@@ -124,12 +189,18 @@ __device__ __forceinline__ int xcd_swizzle(int wg, int nwg)
     return (wg & 7) * (nwg >> 3) + (wg >> 3);
 }
 
+// Occupancy is bounded by LDS to 3 workgroups of 8 waves per CU (R <= 2) or
+// 2 (R = 3), i.e. 6 or 4 waves per SIMD; the register budget is capped to
+// match so that the scheduler does not trade occupancy for hoisted loads.
+template <int R>
+constexpr int waves_per_eu() { return R <= 2 ? 6 : 4; }
+
 template <int R, int WFN>
-__global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
+__global__ __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(waves_per_eu<R>(), waves_per_eu<R>()))) void
+filter_tiled_fp32(TiledArgs a)
 {
     using T = Tile<R>;
-    constexpr Rows<R> rows{};
-    __shared__ float lds[2][T::LDS];
+    __shared__ __attribute__((aligned(16))) float lds[2][T::LDS];
 
     const int t = threadIdx.x;
     const int wg = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -147,7 +218,7 @@ __global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
     float p0[T::CPT][4];
 #pragma unroll
     for (int c = 0; c < T::CPT; c++) {
-        int ci = t + c * NT;
+        int ci = t + c * NTF;
         if (ci >= T::NCHUNK) ci = 0;  // duplicate of chunk 0, never written
         const int cx = ci % T::CPR, row = ci / T::CPR;
         const int ry = row % T::RY, rz = row / T::RY;
@@ -156,9 +227,9 @@ __global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
         for (int e = 0; e < 4; e++) p0[c][e] = Pbase[coff[c] + e];
     }
 
-    // This thread's 8-voxel run.
-    const int xr = t & 3, ty = (t >> 2) & 7, tz = t >> 5;
-    const int x0 = X0 + 8 * xr, y = Y0 + ty, z = Z0 + tz;
+    // This thread's 4-voxel run.
+    const int xr = t & 7, ty = (t >> 3) & 7, tz = t >> 6;
+    const int x0 = X0 + VX * xr, y = Y0 + ty, z = Z0 + tz;
     const long vox = (long)(z + g.pz) * sp + (long)(y + g.py) * g.nxp + (x0 + g.px);
     const float *Rv = a.R + vox;
     const int zc = z < g.nz ? z : g.nz - 1;
@@ -169,14 +240,24 @@ __global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
 #pragma unroll
     for (int v = 0; v < VX; v++) fv[v] = fw[v] = wmax[v] = 0.f;
 
-    const int ldsrow = ((tz + R) * T::RY + (ty + R)) * T::PITCH + 8 * xr;
+    const int ldsrow = ((tz + R) * T::RY + (ty + R)) * T::PITCH + VX * xr;
     const int nS = a.n_search;
 
-    auto stage = [&](float (&pn)[T::CPT][4], float *buf) {
+    auto load_window = [](const float *src, float (&win)[T::WIN]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < T::WIN / 4; i++) {
+            const float4 q = *reinterpret_cast<const float4 *>(src + 4 * i);
+            win[4 * i] = q.x;
+            win[4 * i + 1] = q.y;
+            win[4 * i + 2] = q.z;
+            win[4 * i + 3] = q.w;
+        }
+    };
+    auto stage = [&](float (&pn)[T::CPT][4], float *buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < T::CPT; c++) {
-            const int ci = t + c * NT;
-            if (T::NCHUNK % NT == 0 || ci < T::NCHUNK) {
+            const int ci = t + c * NTF;
+            if (T::NCHUNK % NTF == 0 || ci < T::NCHUNK) {
                 float4 dv;
                 float q;
                 q = pn[c][0] - p0[c][0]; dv.x = q * q;
@@ -187,7 +268,7 @@ __global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
             }
         }
     };
-    auto load_chunks = [&](float (&pn)[T::CPT][4], int lin) {
+    auto load_chunks = [&](float (&pn)[T::CPT][4], int lin) __attribute__((always_inline)) {
         const float *base = Pbase + lin;
 #pragma unroll
         for (int c = 0; c < T::CPT; c++)
@@ -214,35 +295,44 @@ __global__ __launch_bounds__(NT) void filter_tiled_fp32(TiledArgs a)
         const float *L = lds[k & 1] + ldsrow;
         float d[VX];
         {
+            // the centre tap leads the sum (iternlm_prepare.cpp:176-177)
             float win[T::WIN];
+            load_window(L, win);
+            const float w0 = a.wc[0];
 #pragma unroll
-            for (int i = 0; i < T::WIN; i++) win[i] = L[i];
-#pragma unroll
-            for (int v = 0; v < VX; v++) d[v] = win[R + v] * a.w[0];
+            for (int v = 0; v < VX; v++) d[v] = win[R + v] * w0;
         }
-#pragma unroll
-        for (int ri = 0; ri < rows.n; ri++) {
-            const Row rw = rows.r[ri];
-            const float *Lr = L + (rw.dz * T::RY + rw.dy) * T::PITCH;
-            float win[T::WIN];
-#pragma unroll
-            for (int i = 0; i < T::WIN; i++) win[i] = Lr[i];
-            int tap = rw.tap0;
-#pragma unroll
-            for (int dx = -rw.j; dx <= rw.j; dx++) {
-                if (rw.dz == 0 && rw.dy == 0 && dx == 0) continue;
-                const float wt = a.w[tap];
-#pragma unroll
-                for (int v = 0; v < VX; v++) d[v] = d[v] + win[R + v + dx] * wt;
-                tap++;
+        // Rows in reference order; the window of row i+1 is read while row i
+        // is summed.  A compiler barrier per row keeps the optimiser from
+        // hoisting every window above the arithmetic (which exhausts the
+        // register budget and spills).
+        constexpr int NR = RowsOf<R>::value.n;
+        float win[2][T::WIN];
+        load_window(L + (RowsOf<R>::value.r[0].dz * T::RY + RowsOf<R>::value.r[0].dy) * T::PITCH, win[0]);
+        static_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+            constexpr int i = decltype(ri)::value;
+            constexpr Row rw = RowsOf<R>::value.r[i];
+            if constexpr (i + 1 < NR) {
+                constexpr Row nx = RowsOf<R>::value.r[i + 1];
+                load_window(L + (nx.dz * T::RY + nx.dy) * T::PITCH, win[(i + 1) & 1]);
             }
-        }
+            static_for<0, 2 * rw.j + 1>([&](auto kk) __attribute__((always_inline)) {
+                constexpr int dx = decltype(kk)::value - rw.j;
+                if constexpr (!(rw.dz == 0 && rw.dy == 0 && dx == 0)) {
+                    constexpr int cls = RowsOf<R>::value.class_of(dx, rw.dy, rw.dz);
+                    const float wt = a.wc[cls];
+#pragma unroll
+                    for (int v = 0; v < VX; v++) d[v] = d[v] + win[i & 1][R + v + dx] * wt;
+                }
+            });
+            pin(d);
+        });
 #pragma unroll
         for (int v = 0; v < VX; v++) {
             const float wt = weight_of<WFN>(d[v] * m);
             fv[v] = fv[v] + wt * rs[v];
             fw[v] = fw[v] + wt;
-            wmax[v] = (wt > wmax[v]) ? wt : wmax[v];
+            wmax[v] = __builtin_fmaxf(wmax[v], wt);  // == (wt > wmax ? wt : wmax): weights are >= 0, never NaN
         }
         if (more) stage(pn, lds[(k + 1) & 1]);
         __syncthreads();
@@ -309,7 +399,7 @@ __global__ __launch_bounds__(NT) void filter_pervoxel_fp32(VoxelArgs a)
         const float wt = weight_of<WFN>(d * m);
         fv = fv + wt * a.R[i1];
         fw = fw + wt;
-        wmax = (wt > wmax) ? wt : wmax;
+        wmax = __builtin_fmaxf(wmax, wt);
     }
     const float r0 = a.R[i0];
     float res = r0;
@@ -378,7 +468,33 @@ __global__ __launch_bounds__(NT) void patchstd_kernel(const float *__restrict__ 
     if (lane == 0) out[s] = sqrt(var);
 }
 
+// Every negative float from -0 down to the cutoff, as bit patterns
+// 0x80000000 + [first, first + count): the Pade weight with the shortened
+// division against the same weight with IEEE division.
+__global__ __launch_bounds__(NT) void selftest_div_kernel(unsigned first, unsigned count,
+                                                          unsigned long long *mismatch, unsigned *example)
+{
+    const unsigned stride = gridDim.x * NT;
+    unsigned bad = 0;
+    for (unsigned i = blockIdx.x * NT + threadIdx.x; i < count; i += stride) {
+        const float a = __uint_as_float(0x80000000u + first + i);
+        const float x = pade_cut<false>(a), y = pade_cut<true>(a);
+        if (__float_as_uint(x) != __float_as_uint(y)) {
+            bad++;
+            *example = 0x80000000u + first + i;
+        }
+    }
+    if (bad) atomicAdd(mismatch, (unsigned long long)bad);
+}
+
 }  // namespace
+
+hipError_t launch_selftest_div(unsigned first, unsigned count, unsigned long long *mismatch, unsigned *example,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(selftest_div_kernel, dim3(8192), dim3(NT), 0, st, first, count, mismatch, example);
+    return hipGetLastError();
+}
 
 // ---------------------------------------------------------------------------
 // Host launchers
@@ -386,7 +502,7 @@ __global__ __launch_bounds__(NT) void patchstd_kernel(const float *__restrict__ 
 template <int R>
 static bool taps_match_r(const int *dxyz, int n)
 {
-    constexpr Rows<R> rows{};
+    constexpr const Rows<R> &rows = RowsOf<R>::value;
     if (n != rows.ntaps) return false;
     if (dxyz[0] != 0 || dxyz[1] != 0 || dxyz[2] != 0) return false;
     for (int ri = 0; ri < rows.n; ri++) {
@@ -401,12 +517,32 @@ static bool taps_match_r(const int *dxyz, int n)
     return true;
 }
 
-bool tiled_taps_match(int rp, const int *dxyz, int n)
+// Per-class weights of the tiled kernel; false when two taps of one class
+// carry different weights (then the per-voxel kernel must run).
+template <int R>
+static bool class_weights_r(const int *dxyz, const float *w, int n, float *wc)
 {
+    constexpr const Rows<R> &rows = RowsOf<R>::value;
+    bool seen[kMaxClasses] = {};
+    for (int i = 0; i < n; i++) {
+        const int c = rows.class_of(dxyz[3 * i], dxyz[3 * i + 1], dxyz[3 * i + 2]);
+        if (c < 0) return false;
+        if (!seen[c]) {
+            wc[c] = w[i];
+            seen[c] = true;
+        } else if (__builtin_memcmp(&wc[c], &w[i], sizeof(float)) != 0)
+            return false;
+    }
+    return true;
+}
+
+bool tiled_taps_match(int rp, const int *dxyz, const float *w, int n)
+{
+    float wc[kMaxClasses];
     switch (rp) {
-    case 1: return taps_match_r<1>(dxyz, n);
-    case 2: return taps_match_r<2>(dxyz, n);
-    case 3: return taps_match_r<3>(dxyz, n);
+    case 1: return taps_match_r<1>(dxyz, n) && class_weights_r<1>(dxyz, w, n, wc);
+    case 2: return taps_match_r<2>(dxyz, n) && class_weights_r<2>(dxyz, w, n, wc);
+    case 3: return taps_match_r<3>(dxyz, n) && class_weights_r<3>(dxyz, w, n, wc);
     default: return false;
     }
 }
@@ -436,9 +572,10 @@ static hipError_t launch_tiled(const float *Rv, const float *P, float *out, cons
     a.ntx = (g.nx + TX - 1) / TX;
     a.nty = (g.ny + TY - 1) / TY;
     a.ntz = (g.nz + TZ - 1) / TZ;
-    for (int i = 0; i < kMaxTaps; i++) a.w[i] = i < taps.n ? taps.w[i] : 0.f;
+    for (int i = 0; i < kMaxClasses; i++) a.wc[i] = 0.f;
+    if (!class_weights_r<R>(taps.dxyz, taps.w, taps.n, a.wc)) return hipErrorInvalidValue;
     const long nwg = (long)a.ntx * a.nty * a.ntz;
-    hipLaunchKernelGGL((filter_tiled_fp32<R, WFN>), dim3((unsigned)nwg), dim3(NT), 0, st, a);
+    hipLaunchKernelGGL((filter_tiled_fp32<R, WFN>), dim3((unsigned)nwg), dim3(NTF), 0, st, a);
     return hipGetLastError();
 }
 

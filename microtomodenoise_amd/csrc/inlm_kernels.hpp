@@ -25,7 +25,8 @@ constexpr int kMaxTaps = 128;
 
 struct TapTable {
     float w[kMaxTaps];
-    int lin[kMaxTaps];  // linear patch shifts in the padded layout
+    int lin[kMaxTaps];       // linear patch shifts in the padded layout
+    int dxyz[3 * kMaxTaps];  // (dx,dy,dz) per tap
     int n;
 };
 
@@ -42,8 +43,15 @@ hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const 
                          const TapTable &taps, const FilterGeom &g, hipStream_t st);
 
 // Whether the tiled kernel for isotropic radius rp uses exactly the given tap
-// order (dx,dy,dz per tap, centre first).
-bool tiled_taps_match(int rp, const int *dxyz, int n);
+// order (dx,dy,dz per tap, centre first) and whether all taps it groups into
+// one weight class carry the same weight.
+bool tiled_taps_match(int rp, const int *dxyz, const float *w, int n);
+
+// Exhaustive check of the parity kernel's division: counts weights that
+// differ from IEEE division over the float bit patterns 0x80000000+[first,
+// first+count).
+hipError_t launch_selftest_div(unsigned first, unsigned count, unsigned long long *mismatch, unsigned *example,
+                               hipStream_t st);
 
 // Patch standard deviation per sample (one wavefront per sample).  Sample
 // indices are linear in the unpadded volume (nx, nslice = nx*ny); the volume
