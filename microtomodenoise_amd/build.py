@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinlm.so")
 
-SOURCES = ["inlm_kernels.hip", "inlm_capi.cpp", "inlm_tables.cpp"]
+SOURCES = ["inlm_kernels.hip", "inlm_mfma.hip", "inlm_capi.cpp", "inlm_tables.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
@@ -45,5 +45,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_main.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_main")
+
+
+def build_facade_test(verbose: bool = False) -> str:
+    """Compiles the C++ driver of the header-only drop-in facade
+    (include/iternlm_gpu.h) against libinlm.so, for the GPU tests."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), FACADE_SRC, "-L" + HERE, "-linlm",
+           "-Wl,-rpath," + HERE, "-o", FACADE_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the facade driver")
+    return FACADE_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_facade_test(verbose=True))

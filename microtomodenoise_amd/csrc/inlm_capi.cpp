@@ -17,6 +17,7 @@
 namespace {
 
 std::string g_create_error;  // message of the last failed inlm_create
+constexpr size_t kGuard = 64;
 
 }  // namespace
 
@@ -32,7 +33,16 @@ struct inlm_handle {
     inlm::TapTable taps{};
     int rp_tiled = 0;
     size_t pad_elems = 0;  // allocated elements of each padded volume (>= padded voxels)
+    // FP32 parity form: padded original and previous iterate in float.
+    // Matrix-core forms: the same two volumes normalised, in the form's
+    // storage type (Rs/Ps), plus the unpadded original in float.
     float *R_pad = nullptr, *P_pad = nullptr, *result = nullptr, *sigma = nullptr;
+    void *R_alloc = nullptr, *P_alloc = nullptr;
+    void *Rs = nullptr, *Ps = nullptr;
+    float *raw = nullptr;
+    double *moments = nullptr;
+    bool fast = false;
+    double shift = 0.0, scale = 1.0;
     inlm::SearchEntry *search_dev = nullptr;
     int64_t *samples_dev = nullptr;
     double *std_dev = nullptr;
@@ -150,8 +160,8 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         p.radius_searchspace[2] = 0;
         p.radius_patchspace[2] = 0;
     }
-    if (p.precision != INLM_PREC_FP32)
-        return fail(nullptr, INLM_BAD_ARG, "precision form not available in this build");
+    if (p.precision < INLM_PREC_FP32 || p.precision > INLM_PREC_FP16)
+        return fail(nullptr, INLM_BAD_ARG, "unknown precision form");
     if (p.weight_fn != INLM_WEIGHT_PADE && p.weight_fn != INLM_WEIGHT_EXP)
         return fail(nullptr, INLM_BAD_ARG, "unknown weight function");
 
@@ -210,8 +220,17 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         h->tap_dxyz.push_back(PT[i].dz);
     }
     const int *rp = p.radius_patchspace;
-    if (p.kernel_select == 0 && rp[0] == rp[1] && rp[1] == rp[2] && rp[0] >= 1 && rp[0] <= 3 &&
-        inlm::tiled_taps_match(rp[0], h->tap_dxyz.data(), h->taps.w, h->taps.n))
+    const bool iso = rp[0] == rp[1] && rp[1] == rp[2] && rp[0] >= 1 && rp[0] <= 3;
+    h->fast = p.precision != INLM_PREC_FP32;
+    if (h->fast) {
+        // the matrix-core forms cover isotropic patches of radius 1..3 whose
+        // weights depend on (dx^2+dy^2, dz^2) only
+        if (!iso || !inlm::mfma_taps_supported(rp[0], h->tap_dxyz.data(), h->taps.w, h->taps.n)) {
+            delete h;
+            return fail(nullptr, INLM_BAD_ARG, "the matrix-core forms need an isotropic patch of radius 1, 2 or 3");
+        }
+        h->rp_tiled = rp[0];
+    } else if (p.kernel_select == 0 && iso && inlm::tiled_taps_match(rp[0], h->tap_dxyz.data(), h->taps.w, h->taps.n))
         h->rp_tiled = rp[0];
 
     inlm_status st = INLM_OK;
@@ -226,18 +245,32 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
     if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipEventCreate"));
     h->pad_elems = padded_alloc_elems(g);
-    const size_t padb = h->pad_elems * sizeof(float);
-    if ((e = hipMalloc(&h->R_pad, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(original)"));
-    if ((e = hipMalloc(&h->P_pad, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(previous)"));
+    // kGuard elements in front of each padded volume: the matrix-core staging
+    // reads 4-column chunks that start up to 3 elements before the volume.
+    const size_t esz = inlm::form_elem_size(h->p.precision);
+    const size_t padb = (h->pad_elems + kGuard) * esz;
+    if ((e = hipMalloc(&h->R_alloc, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(original)"));
+    if ((e = hipMalloc(&h->P_alloc, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(previous)"));
+    if ((e = hipMemsetAsync(h->R_alloc, 0, padb, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->P_alloc, 0, padb, h->stream)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMemset"));
+    h->Rs = static_cast<char *>(h->R_alloc) + kGuard * esz;
+    h->Ps = static_cast<char *>(h->P_alloc) + kGuard * esz;
+    if (h->fast) {
+        if ((e = hipMalloc(&h->raw, sizeof(float) * g.voxels())) != hipSuccess)
+            return bail(hip_fail(nullptr, e, "hipMalloc(raw)"));
+        if ((e = hipMalloc(&h->moments, 2 * sizeof(double))) != hipSuccess)
+            return bail(hip_fail(nullptr, e, "hipMalloc(moments)"));
+    } else {
+        h->R_pad = static_cast<float *>(h->Rs);
+        h->P_pad = static_cast<float *>(h->Ps);
+    }
     if ((e = hipMalloc(&h->result, sizeof(float) * g.voxels())) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipMalloc(result)"));
     if ((e = hipMalloc(&h->sigma, sizeof(float) * g.nz)) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipMalloc(sigma)"));
     if ((e = hipMalloc(&h->search_dev, sizeof(inlm::SearchEntry) * h->search.size())) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipMalloc(search)"));
-    if ((e = hipMemsetAsync(h->R_pad, 0, padb, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(h->P_pad, 0, padb, h->stream)) != hipSuccess)
-        return bail(hip_fail(nullptr, e, "hipMemset"));
     if ((e = hipMemcpyAsync(h->search_dev, h->search.data(), sizeof(inlm::SearchEntry) * h->search.size(),
                             hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
@@ -252,8 +285,10 @@ inlm_status inlm_destroy(inlm_handle *h)
     if (!h) return INLM_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    (void)hipFree(h->R_pad);
-    (void)hipFree(h->P_pad);
+    (void)hipFree(h->R_alloc);
+    (void)hipFree(h->P_alloc);
+    (void)hipFree(h->raw);
+    (void)hipFree(h->moments);
     (void)hipFree(h->result);
     (void)hipFree(h->sigma);
     (void)hipFree(h->search_dev);
@@ -282,10 +317,29 @@ inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device)
     if (!h || !raw) return fail(h, INLM_BAD_ARG, "null argument");
     HIP_TRY(h, hipSetDevice(h->device));
     const size_t bytes = sizeof(float) * h->geo.voxels();
-    HIP_TRY(h, hipMemcpyAsync(h->result, raw, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                              h->stream));
-    HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->R_pad, h->fg, h->stream));
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (!h->fast) {
+        HIP_TRY(h, hipMemcpyAsync(h->result, raw, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  h->stream));
+        HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->R_pad, h->fg, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+    } else {
+        // normalisation: shift = mean, scale = 1/std of the original (FP64)
+        HIP_TRY(h, hipMemcpyAsync(h->raw, raw, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  h->stream));
+        HIP_TRY(h, hipMemsetAsync(h->moments, 0, 2 * sizeof(double), h->stream));
+        HIP_TRY(h, inlm::launch_moments(h->raw, (long)h->geo.voxels(), h->moments, h->stream));
+        double mom[2];
+        HIP_TRY(h, hipMemcpyAsync(mom, h->moments, sizeof mom, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        const double nvox = (double)h->geo.voxels();
+        const double mean = mom[0] / nvox;
+        const double var = std::max(mom[1] / nvox - mean * mean, 0.0);
+        h->shift = mean;
+        h->scale = var > 0 ? 1.0 / std::sqrt(var) : 1.0;
+        HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->raw, h->Rs, h->fg, (float)h->shift,
+                                              (float)h->scale, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+    }
     h->raw_loaded = true;
     h->last_iter = 0;
     return INLM_OK;
@@ -302,15 +356,26 @@ inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
     }
     if (!h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
     if (!h->sigma_set) return fail(h, INLM_BAD_ARG, "sigma not set");
-    const float *P = h->R_pad;  // iteration 1: the previous iterate is the original
+    const void *P = h->Rs;  // iteration 1: the previous iterate is the original
     if (iter > 1) {
         if (h->last_iter < 1) return fail(h, INLM_BAD_ARG, "iteration > 1 before iteration 1");
-        HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->P_pad, h->fg, h->stream));
-        P = h->P_pad;
+        if (h->fast)
+            HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->result, h->Ps, h->fg, (float)h->shift,
+                                                  (float)h->scale, h->stream));
+        else
+            HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->P_pad, h->fg, h->stream));
+        P = h->Ps;
     }
     HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
-    HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad, P, h->result, h->sigma,
-                                   h->p.beta, h->search_dev, (int)h->search.size(), h->taps, h->fg, h->stream));
+    if (h->fast)
+        HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->raw, h->result,
+                                            h->sigma, h->p.beta, (float)(1.0 / (h->scale * h->scale)), h->shift,
+                                            1.0 / h->scale, h->search_dev, (int)h->search.size(), h->taps, h->fg,
+                                            h->stream));
+    else
+        HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad,
+                                       static_cast<const float *>(P), h->result, h->sigma, h->p.beta, h->search_dev,
+                                       (int)h->search.size(), h->taps, h->fg, h->stream));
     HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     HIP_TRY(h, hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
@@ -360,7 +425,11 @@ inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples,
     HIP_TRY(h, hipMemcpyAsync(h->samples_dev, samples, sizeof(int64_t) * n, hipMemcpyHostToDevice, h->stream));
     const float *origin;
     int64_t rowp, slp;
-    if (which == 0) {
+    if (which == 0 && h->fast) {
+        origin = h->raw;
+        rowp = g.nx;
+        slp = nslice;
+    } else if (which == 0) {
         const long sp = (long)g.nxp * g.nyp;
         origin = h->R_pad + (long)g.pz * sp + (long)g.py * g.nxp + g.px;
         rowp = g.nxp;

@@ -22,6 +22,7 @@ struct FilterGeom {
 enum FilterForm { FORM_FP32 = 0, FORM_FP32_FAST = 1, FORM_BF16 = 2, FORM_FP16 = 3 };
 
 constexpr int kMaxTaps = 128;
+constexpr int kMfmaMaxPatterns = 16;
 
 struct TapTable {
     float w[kMaxTaps];
@@ -59,5 +60,18 @@ hipError_t launch_selftest_div(unsigned first, unsigned count, unsigned long lon
 // slice pitches, so the padded original can be read in place.
 hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice_pitch, const int64_t *samples,
                            int64_t n, int nx, int64_t nslice, int patchsize, double *out, hipStream_t st);
+
+// Matrix-core forms (FP32_FAST, FP16, BF16).  Rs/Ps are normalised padded
+// volumes in the form's storage type; raw is the unpadded original in float
+// (returned unchanged where every weight vanishes).
+bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n);
+hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, const float *raw,
+                              float *out, const float *sigma, float beta, float mscale, double shift,
+                              double inv_scale, const SearchEntry *search, int n_search, const TapTable &taps,
+                              const FilterGeom &g, hipStream_t st);
+hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
+                                hipStream_t st);
+hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);
+inline size_t form_elem_size(int form) { return form == FORM_FP32 || form == FORM_FP32_FAST ? 4 : 2; }
 
 }  // namespace inlm

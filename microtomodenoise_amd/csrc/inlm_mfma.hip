@@ -1,0 +1,510 @@
+// Matrix-core (MFMA) form of the iterNLM filter for gfx950.
+//
+// For a search offset s the patch distance is a small 3D convolution of the
+// non-negative field D_s(y) = (P(y+s) - P(y))^2 with the patch weights:
+//   d_s(x) = sum_rows sum_dx W(dx,dy,dz) D_s(x + (dx,dy,dz)).
+// For one patch row (dz,dy) and a block of 16 outputs along x the row sum is
+// a banded Toeplitz product, so a 16x16 block of outputs (16 consecutive x by
+// 16 (y,z) rows) takes one v_mfma_f32_16x16x32 per patch row:
+//   A[m][k] = D_s[row m + (dy,dz)][x0 - 8 + k]    (16-bit D plane in LDS)
+//   B[k][n] = W(x0 - 8 + k - (x0 + n), dy, dz)     (band of the row's weights)
+//   C[m][n] += A . B                               (FP32 accumulate)
+// D is a sum of non-negative terms (no cancellation), so 16-bit D planes keep
+// the distance to ~1e-3 relative (BF16) or ~1e-4 (FP16).  The weight, the
+// weighted sums and wmax stay FP32; the final normalisation is FP64.  Volumes
+// are stored normalised, v_s = (v - shift) * scale, which leaves every
+// weight unchanged once the per-slice multiplier is divided by scale^2.
+//
+// A workgroup (8 wavefronts) owns a 32x8x8 output tile and stages D_s over
+// the tile plus the patch halo in LDS (double buffered, one barrier per
+// offset); wavefront w computes the 16x16 block x-half (w & 1), z pair (w >> 1).
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "inlm_kernels.hpp"
+
+namespace inlm {
+namespace {
+
+constexpr int NTM = 512;
+constexpr int MTX = 32, MTY = 8, MTZ = 8;
+constexpr int XW = 56;  // LDS row pitch (elements); columns [0,4) and [44,56) stay zero
+constexpr float kCut = -3.56648f;
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Patch rows of the isotropic ellipsoid of radius R and their weight
+// patterns: a row's weights as a function of dx depend only on (dy^2, dz^2),
+// so rows sharing that key share one B operand.
+struct MRow {
+    int dz, dy, j, pat;
+};
+template <int R>
+struct MRows {
+    MRow r[4 * R * R + 4 * R + 1];
+    int n, npat;
+    int pdy[kMfmaMaxPatterns], pdz[kMfmaMaxPatterns], pj[kMfmaMaxPatterns];
+    static constexpr bool inside(int x, int y, int z)
+    {
+        const float rr = (float)R;
+        const float fx = (float)x / rr, fy = (float)y / rr, fz = (float)z / rr;
+        float e = fx * fx;
+        e = e + fy * fy;
+        e = e + fz * fz;
+        return e <= 1.f;
+    }
+    constexpr MRows() : r(), n(0), npat(0), pdy(), pdz(), pj()
+    {
+        for (int z = -R; z <= R; ++z)
+            for (int y = -R; y <= R; ++y) {
+                int j = -1;
+                for (int x = 0; x <= R; ++x)
+                    if (inside(x, y, z)) j = x;
+                if (j < 0) continue;
+                int p = -1;
+                for (int q = 0; q < npat; q++)
+                    if (pdy[q] == (y < 0 ? -y : y) && pdz[q] == (z < 0 ? -z : z)) p = q;
+                if (p < 0) {
+                    p = npat++;
+                    pdy[p] = y < 0 ? -y : y;
+                    pdz[p] = z < 0 ? -z : z;
+                    pj[p] = j;
+                }
+                r[n++] = MRow{z, y, j, p};
+            }
+    }
+};
+template <int R>
+struct MRowsOf {
+    static constexpr MRows<R> value{};
+    static_assert(value.npat <= kMfmaMaxPatterns, "too many row patterns");
+};
+
+template <int I, int N, class F>
+__device__ __forceinline__ void mstatic_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        mstatic_for<I + 1, N>(f);
+    }
+}
+
+// 16-bit D plane element type and the matching MFMA.
+template <class TD>
+struct DPlane;
+template <>
+struct DPlane<_Float16> {
+    typedef half8 vec8;
+    static __device__ __forceinline__ _Float16 from(float v) { return (_Float16)v; }
+    static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct DPlane<__bf16> {
+    typedef bf16x8 vec8;
+    static __device__ __forceinline__ __bf16 from(float v) { return (__bf16)v; }
+    static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <class TS>
+struct MfmaArgs {
+    const TS *R;
+    const TS *P;
+    const float *raw;  // unpadded original
+    float *out;
+    const float *sigma;
+    const SearchEntry *search;
+    int n_search;
+    float beta;
+    float mscale;      // 1 / scale^2: distances in stored units -> data units
+    double shift;      // data = shift + stored * inv_scale
+    double inv_scale;
+    FilterGeom g;
+    int ntx, nty, ntz;
+    float band[kMfmaMaxPatterns][8];  // band[p][dx + R]: weight of pattern p at dx
+};
+
+template <int R>
+struct MTile {
+    static constexpr int RY = MTY + 2 * R, RZ = MTZ + 2 * R;
+    static constexpr int CPR = 10;  // 4-column chunks staged per row: columns [4, 44)
+    static constexpr int NCHUNK = RZ * RY * CPR;
+    static constexpr int CPT = (NCHUNK + NTM - 1) / NTM;
+    static constexpr int LDS = RZ * RY * XW;  // elements per buffer
+};
+
+__device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
+{
+    if ((nwg & 7) != 0) return wg;
+    return (wg & 7) * (nwg >> 3) + (wg >> 3);
+}
+
+template <int WFN>
+__device__ __forceinline__ float fast_weight(float a)
+{
+    if constexpr (WFN == 0) {
+        // Pade [3/3] in Horner form with fused multiply-adds and a hardware
+        // reciprocal: the reference function to a few ulp, not bit-identical.
+        const float num = __builtin_fmaf(__builtin_fmaf(a + 12.f, a, 60.f), a, 120.f);
+        const float den = __builtin_fmaf(__builtin_fmaf(12.f - a, a, -60.f), a, 120.f);
+        const float w = num * __builtin_amdgcn_rcpf(den);
+        return a > kCut ? w : 0.f;
+    } else {
+        return __builtin_amdgcn_exp2f(a * 1.4426950408889634f);
+    }
+}
+
+template <class TS>
+__device__ __forceinline__ void load4(const TS *p, float (&v)[4])
+{
+    if constexpr (sizeof(TS) == 4) {
+        float4 q;
+        __builtin_memcpy(&q, p, 16);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        TS q[4];
+        __builtin_memcpy(q, p, 8);
+#pragma unroll
+        for (int e = 0; e < 4; e++) v[e] = (float)q[e];
+    }
+}
+
+template <int R, class TS, class TD, int WFN>
+__global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
+{
+    using T = MTile<R>;
+    using DP = DPlane<TD>;
+    typedef typename DP::vec8 vec8;
+    constexpr const MRows<R> &rows = MRowsOf<R>::value;
+    __shared__ __attribute__((aligned(16))) TD lds[2][T::LDS];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wg = mxcd_swizzle(blockIdx.x, gridDim.x);
+    const int bx = wg % a.ntx;
+    const int rem = wg / a.ntx;
+    const int by = rem % a.nty;
+    const int bz = rem / a.nty;
+    const int X0 = bx * MTX, Y0 = by * MTY, Z0 = bz * MTZ;
+    const FilterGeom g = a.g;
+    const long sp = (long)g.nxp * g.nyp;
+
+    // Zero both buffers: the guard columns are never staged and must hold
+    // zeros (they meet zero weights in the band, and 0 * NaN would poison).
+    for (int i = t; i < T::LDS; i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
+
+    // B operands, one band per pattern.  Lane l holds B[k][n] for n = l % 16
+    // and k = 8*(l/16) + i, i.e. input column x0 - 8 + k, so dx = k - 8 - n.
+    const int n = lane & 15, q = lane >> 4;
+    vec8 bop[rows.npat];
+#pragma unroll
+    for (int p = 0; p < rows.npat; p++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int dx = 8 * q + i - 8 - n;
+            const float w = (dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f;
+            bop[p][i] = DP::from(w);
+        }
+    }
+
+    // Staging chunks: 4 columns of one region row.  Region row (rz, ry) is
+    // slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
+    const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
+    int coff[T::CPT], loff[T::CPT];
+    float p0[T::CPT][4];
+#pragma unroll
+    for (int c = 0; c < T::CPT; c++) {
+        int ci = t + c * NTM;
+        if (ci >= T::NCHUNK) ci = 0;
+        const int cx = ci % T::CPR, row = ci / T::CPR;
+        const int ry = row % T::RY, rz = row / T::RY;
+        coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
+        loff[c] = row * XW + 4 + 4 * cx;
+        load4(Pt + coff[c], p0[c]);
+    }
+
+    // This lane's outputs: C rows m = 4q + i  ->  (z pair member q/2, y 4(q%2)+i).
+    const int xh = wave & 1, gz = wave >> 1;
+    const int x = X0 + 16 * xh + n;
+    const int z = Z0 + 2 * gz + (q >> 1);
+    const int yb = Y0 + 4 * (q & 1);
+    const long vox = (long)(z + g.pz) * sp + (long)(yb + g.py) * g.nxp + (x + g.px);
+    const TS *Rv = a.R + vox;
+    const int zc = z < g.nz ? z : g.nz - 1;
+    const float sig = a.sigma[zc];
+    const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
+    // A operand: lane l reads row m = l % 16 (slice pair member m/8, row m%8),
+    // columns 16*xh + 8*q .. +7.
+    const int am = lane & 15;
+    const int aoff = ((2 * gz + (am >> 3) + R) * T::RY + ((am & 7) + R)) * XW + 16 * xh + 8 * q;
+
+    float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f}, wmax[4] = {0.f, 0.f, 0.f, 0.f};
+
+    auto load_chunks = [&](float (&pn)[T::CPT][4], int lin) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < T::CPT; c++) load4(Pt + lin + coff[c], pn[c]);
+    };
+    auto stage = [&](const float (&pn)[T::CPT][4], TD *buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < T::CPT; c++) {
+            if (T::NCHUNK % NTM == 0 || t + c * NTM < T::NCHUNK) {
+                TD dv[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float df = pn[c][e] - p0[c][e];
+                    dv[e] = DP::from(__builtin_fminf(df * df, 60000.f));
+                }
+                __builtin_memcpy(buf + loff[c], dv, 8);
+            }
+        }
+    };
+
+    __syncthreads();  // zeroed LDS visible before the first staging
+    {
+        float pn[T::CPT][4];
+        load_chunks(pn, a.search[0].lin);
+        stage(pn, lds[0]);
+    }
+    __syncthreads();
+
+    const int nS = a.n_search;
+    for (int k = 0; k < nS; k++) {
+        const int lin = a.search[k].lin;
+        const bool more = (k + 1) < nS;
+        float pn[T::CPT][4];
+        if (more) load_chunks(pn, a.search[k + 1].lin);
+        float rv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) rv[i] = (float)Rv[(long)i * g.nxp + lin];
+
+        const TD *L = lds[k & 1] + aoff;
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        mstatic_for<0, rows.n>([&](auto ri) __attribute__((always_inline)) {
+            constexpr MRow rw = MRowsOf<R>::value.r[decltype(ri)::value];
+            const vec8 av = *reinterpret_cast<const vec8 *>(L + (rw.dz * T::RY + rw.dy) * XW);
+            acc = DP::mfma(av, bop[rw.pat], acc);
+        });
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float w = fast_weight<WFN>(acc[i] * mm);
+            fv[i] = __builtin_fmaf(w, rv[i], fv[i]);
+            fw[i] = fw[i] + w;
+            wmax[i] = __builtin_fmaxf(wmax[i], w);
+        }
+        if (more) stage(pn, lds[(k + 1) & 1]);
+        __syncthreads();
+    }
+
+    if (x < g.nx && z < g.nz) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int y = yb + i;
+            if (y < g.ny) {
+                const long o = ((long)z * g.ny + y) * g.nx + x;
+                float res = a.raw[o];
+                if (wmax[i] > 0.f) {
+                    const double r0 = (double)(float)Rv[(long)i * g.nxp];
+                    const double v = ((double)fv[i] + (double)wmax[i] * r0) / ((double)fw[i] + (double)wmax[i]);
+                    res = (float)(a.shift + v * a.inv_scale);
+                }
+                a.out[o] = res;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Normalising reflect-pad: padded v_s = (v - shift) * scale in the storage
+// type, mirror without repeating the edge (iternlm_prepare.cpp:6-48).
+// ---------------------------------------------------------------------------
+template <class TS>
+__global__ __launch_bounds__(256) void pad_normalise_kernel(const float *__restrict__ in, TS *__restrict__ out,
+                                                            FilterGeom g, float shift, float scale)
+{
+    const long nn = (long)g.nxp * g.nyp * g.nzp;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nn) return;
+    const int x1 = (int)(idx % g.nxp);
+    const int y1 = (int)((idx / g.nxp) % g.nyp);
+    const int z1 = (int)(idx / ((long)g.nxp * g.nyp));
+    int xx = x1 - g.px, yy = y1 - g.py, zz = z1 - g.pz;
+    while (xx < 0 || xx >= g.nx) xx = (xx < 0) ? -xx : 2 * g.nx - xx - 2;
+    while (yy < 0 || yy >= g.ny) yy = (yy < 0) ? -yy : 2 * g.ny - yy - 2;
+    while (zz < 0 || zz >= g.nz) zz = (zz < 0) ? -zz : 2 * g.nz - zz - 2;
+    const float v = in[((long)zz * g.ny + yy) * g.nx + xx];
+    out[idx] = (TS)((v - shift) * scale);
+}
+
+// Sum and sum of squares of a float volume in FP64 (one atomic pair per
+// workgroup).
+__global__ __launch_bounds__(256) void moments_kernel(const float *__restrict__ in, long n, double *acc)
+{
+    double s = 0.0, s2 = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = in[i];
+        s += v;
+        s2 += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    __shared__ double red[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = s;
+        red[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
+template <int R>
+bool mfma_bands_r(const int *dxyz, const float *w, int ntap, float (*band)[8])
+{
+    constexpr const MRows<R> &rows = MRowsOf<R>::value;
+    bool seen[kMfmaMaxPatterns][8] = {};
+    for (int p = 0; p < kMfmaMaxPatterns; p++)
+        for (int i = 0; i < 8; i++) band[p][i] = 0.f;
+    for (int i = 0; i < ntap; i++) {
+        const int dx = dxyz[3 * i], dy = dxyz[3 * i + 1], dz = dxyz[3 * i + 2];
+        int pat = -1;
+        for (int r = 0; r < rows.n; r++)
+            if (rows.r[r].dy == dy && rows.r[r].dz == dz) pat = rows.r[r].pat;
+        if (pat < 0 || dx < -rows.pj[pat] || dx > rows.pj[pat]) return false;
+        if (!seen[pat][dx + R]) {
+            band[pat][dx + R] = w[i];
+            seen[pat][dx + R] = true;
+        } else if (band[pat][dx + R] != w[i])
+            return false;
+    }
+    // every band position of every pattern must be covered
+    for (int p = 0; p < rows.npat; p++)
+        for (int dx = -rows.pj[p]; dx <= rows.pj[p]; dx++)
+            if (!seen[p][dx + R]) return false;
+    int count = 0;
+    for (int r = 0; r < rows.n; r++) count += 2 * rows.r[r].j + 1;
+    return count == ntap;
+}
+
+template <int R, class TS, class TD, int WFN>
+hipError_t launch_mfma_t(const void *Rs, const void *Ps, const float *raw, float *out, const float *sigma, float beta,
+                         float mscale, double shift, double inv_scale, const SearchEntry *search, int n_search,
+                         const TapTable &taps, const FilterGeom &g, hipStream_t st)
+{
+    MfmaArgs<TS> a;
+    a.R = static_cast<const TS *>(Rs);
+    a.P = static_cast<const TS *>(Ps);
+    a.raw = raw;
+    a.out = out;
+    a.sigma = sigma;
+    a.search = search;
+    a.n_search = n_search;
+    a.beta = beta;
+    a.mscale = mscale;
+    a.shift = shift;
+    a.inv_scale = inv_scale;
+    a.g = g;
+    a.ntx = (g.nx + MTX - 1) / MTX;
+    a.nty = (g.ny + MTY - 1) / MTY;
+    a.ntz = (g.nz + MTZ - 1) / MTZ;
+    if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
+    const long nwg = (long)a.ntx * a.nty * a.ntz;
+    hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int R, int WFN>
+hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, const float *raw, float *out,
+                            const float *sigma, float beta, float mscale, double shift, double inv_scale,
+                            const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
+                            hipStream_t st)
+{
+    switch (form) {
+    case FORM_FP32_FAST:
+        return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
+                                                      search, n_search, taps, g, st);
+    case FORM_FP16:
+        return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
+                                                         search, n_search, taps, g, st);
+    case FORM_BF16:
+        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
+                                                     search, n_search, taps, g, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
+{
+    float band[kMfmaMaxPatterns][8];
+    switch (rp) {
+    case 1: return mfma_bands_r<1>(dxyz, w, n, band);
+    case 2: return mfma_bands_r<2>(dxyz, w, n, band);
+    case 3: return mfma_bands_r<3>(dxyz, w, n, band);
+    default: return false;
+    }
+}
+
+hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, const float *raw,
+                              float *out, const float *sigma, float beta, float mscale, double shift,
+                              double inv_scale, const SearchEntry *search, int n_search, const TapTable &taps,
+                              const FilterGeom &g, hipStream_t st)
+{
+#define INLM_MF(RR)                                                                                               \
+    return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale, \
+                                                    search, n_search, taps, g, st)                                 \
+                          : launch_mfma_form<RR, 1>(form, Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale, \
+                                                    search, n_search, taps, g, st)
+    switch (rp) {
+    case 1: INLM_MF(1);
+    case 2: INLM_MF(2);
+    case 3: INLM_MF(3);
+    default: return hipErrorInvalidValue;
+    }
+#undef INLM_MF
+}
+
+hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
+                                hipStream_t st)
+{
+    const long nn = (long)g.nxp * g.nyp * g.nzp;
+    const unsigned blocks = (unsigned)((nn + 255) / 256);
+    switch (form) {
+    case FORM_FP32_FAST:
+        hipLaunchKernelGGL(pad_normalise_kernel<float>, dim3(blocks), dim3(256), 0, st, in, (float *)out, g, shift,
+                           scale);
+        break;
+    case FORM_FP16:
+        hipLaunchKernelGGL(pad_normalise_kernel<_Float16>, dim3(blocks), dim3(256), 0, st, in, (_Float16 *)out, g,
+                           shift, scale);
+        break;
+    case FORM_BF16:
+        hipLaunchKernelGGL(pad_normalise_kernel<__bf16>, dim3(blocks), dim3(256), 0, st, in, (__bf16 *)out, g, shift,
+                           scale);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st)
+{
+    hipLaunchKernelGGL(moments_kernel, dim3(1024), dim3(256), 0, st, in, n, acc);
+    return hipGetLastError();
+}
+
+}  // namespace inlm

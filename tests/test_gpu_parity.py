@@ -164,3 +164,57 @@ def test_division_selftest_exhaustive(gpu_available):
     _capi.check(_capi.lib().inlm_selftest_division(0, C.byref(mism), C.byref(checked), C.byref(ex)))
     assert checked.value > 1_000_000_000
     assert mism.value == 0, f"{mism.value} mismatches, e.g. bits {ex.value:#x}"
+
+
+def test_cpp_facade_drop_in(gpu_available, tmp_path):
+    """The header-only C++ drop-in (include/iternlm_gpu.h), driven like the
+    reference's main.cpp, reproduces the reference fixture bit for bit."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "facade_main")
+    assert os.path.exists(exe), "tests/cpp/facade_main is not built (run __graft_entry__.build())"
+    f = load_npz("filter_p222_s3_f32.npz")
+    nz, ny, nx = f["raw"].shape
+    f["raw"].astype(np.float32).tofile(tmp_path / "raw.bin")
+    f["sigma1"].astype(np.float32).tofile(tmp_path / "sigma1.bin")
+    f["sigma2"].astype(np.float32).tofile(tmp_path / "sigma2.bin")
+    print("[gpu] C++ facade", flush=True)
+    r = subprocess.run([exe, str(tmp_path), str(nx), str(ny), str(nz), str(int(f["rs"][0])), str(int(f["rp"][0])),
+                        "0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out1 = np.fromfile(tmp_path / "out1.bin", np.float32).reshape(nz, ny, nx)
+    out2 = np.fromfile(tmp_path / "out2.bin", np.float32).reshape(nz, ny, nx)
+    assert np.array_equal(bits(out1), bits(f["out1"]))
+    assert np.array_equal(bits(out2), bits(f["out2"]))
+
+
+# Error budgets of the matrix-core forms against the FP64 oracle of the same
+# algorithm, in units of the phantom's white-noise sigma (one iteration, same
+# sigma list).  FP32_FAST differs by FP16 D planes and fused arithmetic; FP16
+# and BF16 also store the volumes in 16 bits (BF16 keeps 8 significant bits,
+# so its budget is the loosest).
+FAST_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.08, 0.006), "bf16": (0.6, 0.06)}
+
+
+@pytest.mark.parametrize("form", ["fp32_fast", "fp16", "bf16"])
+@pytest.mark.parametrize("rp,rs", [((1, 1, 1), (3, 3, 3)), ((2, 2, 2), (3, 3, 3)), ((3, 3, 3), (2, 2, 2))])
+def test_fast_forms_vs_fp64_oracle(gpu_available, form, rp, rs):
+    spec = ph.PhantomSpec((40, 30, 20), seed=51, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8, streaks=3,
+                          streak_len=(5, 15))
+    raw = ph.make(spec)
+    nz, ny, nx = raw.shape
+    sig = np.full(nz, 700.0, np.float32)
+    print(f"[gpu] fast form {form} rp={rp} rs={rs}", flush=True)
+    g, _ = make_gpu((nx, ny, nz), rs, rp, precision=form)
+    g.set_sigma(sig)
+    g.Run_GaussianNoise(1, raw)
+    got = g.get_result().astype(np.float64)
+    g.free_device()
+    p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False)
+    ref = O.filter_iteration(raw, raw, sig.astype(np.float64), p, precision="f64")
+    err = np.abs(got - ref) / spec.noise_sigma
+    mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
+    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
+    bmax, brms = FAST_BUDGET[form]
+    assert mx <= bmax and rms <= brms, (mx, rms)
