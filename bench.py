@@ -108,9 +108,12 @@ def main():
     cfg, rs, rp, desc = WORKLOADS[args.workload]
     spec = ph.CONFIGS[cfg]
     nx, ny, nz = spec.shape
-    print(f"[bench] generating {cfg} phantom {spec.shape} on the GPU", file=sys.stderr, flush=True)
+    print(f"[bench] generating {cfg} phantom {spec.shape} (geometry on the host, noise on the GPU)", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
     vol = ph.make(spec, backend="torch", device="cuda")
     torch.cuda.synchronize()
+    print(f"[bench] phantom ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     params = DenoiseParameters(radius_searchspace=[rs] * 3, radius_patchspace=[rp] * 3)
     params.noiselevel.mode = "global"
@@ -126,12 +129,14 @@ def main():
     g.set_sigma(s0)
     print(f"[bench] sigma0 {s0[0]:.3f}; iteration 1", file=sys.stderr, flush=True)
     g.Run_GaussianNoise(1)
+    print(f"[bench] iteration 1 filter {g.last_filter_ms():.1f} ms", file=sys.stderr, flush=True)
     s1 = g.estimate_noise(samples, 1)
     sig = (np.float32(params.alpha) * s0 + (np.float32(1) - np.float32(params.alpha)) * s1).astype(np.float32)
     g.set_sigma(sig)
     it = 2
     for _ in range(args.warmup):
         g.Run_GaussianNoise(it)
+        print(f"[bench] warmup step, filter {g.last_filter_ms():.1f} ms", file=sys.stderr, flush=True)
         it += 1
     torch.cuda.synchronize()
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
@@ -140,6 +145,7 @@ def main():
     for _ in range(args.steps):
         g.Run_GaussianNoise(it)
         kms.append(g.last_filter_ms())
+        print(f"[bench]   step done, filter {kms[-1]:.1f} ms", file=sys.stderr, flush=True)
         it += 1
     torch.cuda.synchronize()
     t1 = time.perf_counter()

@@ -197,24 +197,30 @@ def _noise_numpy(spec: PhantomSpec, shape_zyx):
     return n
 
 
+def _blur_axis_torch(t, k, axis):
+    """Correlate along one axis with symmetric (edge-repeating) boundaries,
+    as scipy.ndimage mode="reflect", using shifted adds only."""
+    import torch
+    r = (len(k) - 1) // 2
+    n = t.shape[axis]
+    left = t.narrow(axis, 0, r).flip(axis)
+    right = t.narrow(axis, n - r, r).flip(axis)
+    tp = torch.cat([left, t, right], dim=axis)
+    out = torch.zeros_like(t)
+    for i in range(2 * r + 1):
+        out.add_(tp.narrow(axis, i, n), alpha=float(k[i]))
+    return out
+
+
 def _noise_torch(spec: PhantomSpec, shape_zyx, device):
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(spec.seed + 0x5EED)
     n = torch.randn(shape_zyx, generator=g, device=device, dtype=torch.float32)
     if spec.blur_sigma > 0:
-        k = torch.from_numpy(_gauss_kernel(spec.blur_sigma)).to(device)
-        r = (len(k) - 1) // 2
-        nz, ny, nx = shape_zyx
+        k = _gauss_kernel(spec.blur_sigma)
         for ax in range(3):
-            # move the axis last, convolve rows with reflect padding, move back
-            t = n.movedim(ax, -1).contiguous()
-            sh = t.shape
-            t = t.reshape(-1, 1, sh[-1])
-            t = torch.nn.functional.pad(t, (r, r), mode="reflect")
-            t = torch.nn.functional.conv1d(t, k.view(1, 1, -1))
-            n = t.reshape(sh).movedim(-1, ax)
-        n = n.contiguous()
+            n = _blur_axis_torch(n, k, ax)
     return n
 
 

@@ -194,7 +194,7 @@ def test_cpp_facade_drop_in(gpu_available, tmp_path):
 # sigma list).  FP32_FAST differs by FP16 D planes and fused arithmetic; FP16
 # and BF16 also store the volumes in 16 bits (BF16 keeps 8 significant bits,
 # so its budget is the loosest).
-FAST_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.08, 0.006), "bf16": (0.6, 0.06)}
+FAST_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.08, 0.006), "bf16": (1.0, 0.06)}
 
 
 @pytest.mark.parametrize("form", ["fp32_fast", "fp16", "bf16"])
