@@ -1,22 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the iterNLM hot path on one MI355X.
+"""Benchmark of the iterNLM hot path on one MI355X: plain FP32 NLM against the
+16-bit matrix-core form on the same volume (the metric's two sides).
 
 A step is one filter iteration over the whole resident volume: reflect-pad
-the previous iterate, then one pass of the filter kernel (iterations >= 2 of
-main.cpp:453-545).  Inputs are resident in HBM before the timed region: the
-synthetic phantom is generated on the GPU, loaded into the handle, its noise
-level estimated, and iteration 1 run, all untimed.  value = voxels * steps /
-wall time of the timed steps, in Mvoxel-iter/s.
+the previous iterate, then one pass of the filter kernel (main.cpp:453-545).
+Inputs are resident in HBM before any timed region: the synthetic phantom is
+generated on the GPU, its noise level estimated and the sigma schedule
+(sigma0, then alpha-blended sigma1) fixed once, so both forms run the same 4
+iterations (W untimed + K timed) with the same sigma lists.  Each form's
+value = voxels * K / wall time of its K timed steps, in Mvoxel-iter/s.  The
+reported `value` is the accelerated form's throughput when its PSNR against
+the clean phantom is within 0.05 dB of the plain path's, else the plain
+path's.
 
-The dominant kernel's roofline is computed from HIP events the library
-records on its own stream around each filter launch: algorithmic FLOPs per
-voxel-iteration F = N_s * (2*N_p + 8) (difference and square per offset, a
-multiply and an add per tap, scale, weight, two accumulations, max) against
-the FP32 vector peak of 157.3 TFLOP/s.
+Rooflines come from HIP events the library records on its own stream around
+each filter launch: algorithmic FLOPs per voxel-iteration F = N_s*(2*N_p+8)
+(difference and square per offset, a multiply and an add per tap, scale,
+weight, two accumulations, max) against the 157.3 TFLOP/s FP32 peak.
 
 The CPU baseline is the reference's own CPU kernel (oracle/_ref, compiled
 from its sources) when present, else the oracle restatement, on a bounded
-z-slab sample of the same volume, on the host's cores.
+slab of the same volume, on the host's cores.
 """
 import argparse
 import json
@@ -85,14 +89,58 @@ def cpu_baseline(raw_host, rs, rp, sigma, target_s=12.0):
                       f", {cores} OpenMP threads, {dt:.1f} s"}
 
 
+def run_form(form, vol, nx, ny, nz, rs, rp, sig0, sig1, steps, warmup, log):
+    """W warmup and K timed filter iterations of one arithmetic form on the
+    resident volume with the given sigma lists; returns the result tensor,
+    wall seconds of the timed steps and the filter kernel times."""
+    import torch
+
+    from microtomodenoise_amd import DenoiseParameters, IterativeNLM_GPU
+    params = DenoiseParameters(radius_searchspace=[rs] * 3, radius_patchspace=[rp] * 3, precision=form)
+    params.cpu.blockwise = False
+    g = IterativeNLM_GPU()
+    rc = g.configure_device([nx, ny, nz], params)
+    if rc < 0:
+        raise SystemExit(f"configure_device failed ({rc})")
+    n_search, n_patch = g.table_sizes()
+    g.load_raw_device(vol.data_ptr())
+    it = 1
+    for _ in range(warmup):
+        g.set_sigma(sig0 if it == 1 else sig1)
+        g.Run_GaussianNoise(it)
+        log(f"[bench] {form}: warmup iteration {it}, filter {g.last_filter_ms():.1f} ms")
+        it += 1
+    g.set_sigma(sig1 if it > 1 else sig0)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if it == 2 and warmup == 0:
+            g.set_sigma(sig1)
+        g.Run_GaussianNoise(it)
+        kms.append(g.last_filter_ms())
+        it += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"[bench] {form}: {steps} timed iterations, filter {', '.join(f'{k:.1f}' for k in kms)} ms")
+    out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+    g.get_result_device(out.data_ptr())
+    g.free_device()
+    return out, wall, kms, n_search, n_patch
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
+    ap.add_argument("--form", default="bf16", choices=["bf16", "fp16", "fp32_fast"],
+                    help="accelerated form measured against the plain FP32 path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the filter shards by z-slabs with a "
@@ -105,70 +153,92 @@ def main():
     from microtomodenoise_amd import DenoiseParameters, IterativeNLM_GPU, _capi
     from microtomodenoise_amd import phantom as ph
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if args.copy_probe:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        b = a.clone()
+        torch.cuda.synchronize()
+        del a, b
+        log("[bench] copy probe: 1 GiB read + 1 GiB written")
+
     cfg, rs, rp, desc = WORKLOADS[args.workload]
     spec = ph.CONFIGS[cfg]
     nx, ny, nz = spec.shape
-    print(f"[bench] generating {cfg} phantom {spec.shape} (geometry on the host, noise on the GPU)", file=sys.stderr,
-          flush=True)
+    log(f"[bench] generating {cfg} phantom {spec.shape} (geometry on the host, noise on the GPU)")
     t0 = time.perf_counter()
-    vol = ph.make(spec, backend="torch", device="cuda")
+    vol, clean = ph.make(spec, backend="torch", device="cuda", with_clean=True)
     torch.cuda.synchronize()
-    print(f"[bench] phantom ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    log(f"[bench] phantom ready in {time.perf_counter() - t0:.1f} s")
 
+    # sigma schedule of main.cpp:325-448 from the plain path's estimator:
+    # sigma0 on the original, sigma1 blended after iteration 1
     params = DenoiseParameters(radius_searchspace=[rs] * 3, radius_patchspace=[rp] * 3)
     params.noiselevel.mode = "global"
     params.cpu.blockwise = False
     g = IterativeNLM_GPU()
-    rc = g.configure_device([nx, ny, nz], params)
-    if rc < 0:
-        raise SystemExit(f"configure_device failed ({rc})")
-    n_search, n_patch = g.table_sizes()
+    if g.configure_device([nx, ny, nz], params) < 0:
+        raise SystemExit("configure_device failed")
     g.load_raw_device(vol.data_ptr())
     samples = _capi.draw_samples([nx, ny, nz], params.noiselevel.n_samples, params.noiselevel.patchsize, 0)
     s0 = g.estimate_noise(samples, 0)
     g.set_sigma(s0)
-    print(f"[bench] sigma0 {s0[0]:.3f}; iteration 1", file=sys.stderr, flush=True)
     g.Run_GaussianNoise(1)
-    print(f"[bench] iteration 1 filter {g.last_filter_ms():.1f} ms", file=sys.stderr, flush=True)
     s1 = g.estimate_noise(samples, 1)
-    sig = (np.float32(params.alpha) * s0 + (np.float32(1) - np.float32(params.alpha)) * s1).astype(np.float32)
-    g.set_sigma(sig)
-    it = 2
-    for _ in range(args.warmup):
-        g.Run_GaussianNoise(it)
-        print(f"[bench] warmup step, filter {g.last_filter_ms():.1f} ms", file=sys.stderr, flush=True)
-        it += 1
-    torch.cuda.synchronize()
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.Run_GaussianNoise(it)
-        kms.append(g.last_filter_ms())
-        print(f"[bench]   step done, filter {kms[-1]:.1f} ms", file=sys.stderr, flush=True)
-        it += 1
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    g.free_device()
+    sig1 = (np.float32(params.alpha) * s0 + (np.float32(1) - np.float32(params.alpha)) * s1).astype(np.float32)
+    log(f"[bench] sigma0 {s0[0]:.3f}, sigma1 {sig1[0]:.3f}")
+
+    out_p, wall_p, kms_p, n_search, n_patch = run_form("fp32", vol, nx, ny, nz, rs, rp, s0, sig1, args.steps,
+                                                       args.warmup, log)
+    out_f, wall_f, kms_f, _, _ = run_form(args.form, vol, nx, ny, nz, rs, rp, s0, sig1, args.steps, args.warmup,
+                                          log)
     V = nx * ny * nz
-    value = V * args.steps / wall / 1e6
-    kern_ms = float(np.mean(kms))
+    clean_t = torch.from_numpy(clean).to("cuda")
+    peak = 65535.0
+    mse_p = float(torch.mean((out_p.double() - clean_t.double()) ** 2))
+    mse_f = float(torch.mean((out_f.double() - clean_t.double()) ** 2))
+    mse_n = float(torch.mean((vol.double() - clean_t.double()) ** 2))
+    psnr = lambda m: 10.0 * np.log10(peak * peak / m)  # noqa: E731
+    d_psnr = psnr(mse_p) - psnr(mse_f)
+    diff = (out_f.double() - out_p.double())
+    rmse_fp = float(torch.sqrt(torch.mean(diff ** 2)))
+    maxabs_fp = float(torch.max(torch.abs(diff)))
+    del diff, clean_t
+    log(f"[bench] PSNR noisy {psnr(mse_n):.3f} dB, plain {psnr(mse_p):.3f} dB, {args.form} {psnr(mse_f):.3f} dB "
+        f"(loss {d_psnr:.4f} dB); {args.form} vs plain RMSE {rmse_fp:.3f} max {maxabs_fp:.3f}")
+
+    v_plain = V * args.steps / wall_p / 1e6
+    v_fast = V * args.steps / wall_f / 1e6
+    within = d_psnr <= 0.05
     F = n_search * (2 * n_patch + 8)
-    achieved = F * V / (kern_ms * 1e-3) / 1e12
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"filter_fp32_{args.workload}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    def roof(kernel, kms):
+        km = float(np.mean(kms))
+        ach = F * V / (km * 1e-3) / 1e12
+        return {"bound": "valu", "kernel": kernel, "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4), "flops_per_voxel_iter": F,
+                "kernel_ms": round(km, 3), "traffic": traffic_of(kernel)}
 
+    def traffic_of(kernel):
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                return json.load(fh).get(kernel)
+        return None
+
+    fast_kernel = f"filter_mfma<{rp},{args.form}>"
+    plain_kernel = f"filter_tiled_fp32<{rp}>"
     cpu = None
     if not args.no_cpu_baseline:
         raw_host = vol.cpu().numpy()
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(raw_host, rs, rp, float(sig[nz // 2]), args.cpu_seconds)
-    g.free_device()
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(raw_host, rs, rp, float(sig1[nz // 2]), args.cpu_seconds)
 
+    value = v_fast if within else v_plain
+    wall = wall_f if within else wall_p
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -180,14 +250,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16" if (within and args.form == "bf16") else ("f16" if within and args.form == "fp16" else "f32"),
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "shape_xyz": [nx, ny, nz], "patch_radius": rp,
-                   "search_radius": rs, "n_offsets": n_search, "n_taps": n_patch, "form": "fp32 parity",
-                   "iterations_per_step": 1, "noise": "global, sigma from the GPU estimator"},
-        "roofline": {"bound": "valu", "kernel": f"filter_tiled_fp32<{rp}>", "achieved": round(achieved, 3),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "flops_per_voxel_iter": F, "kernel_ms": round(kern_ms, 3), "traffic": traffic},
+                   "search_radius": rs, "n_offsets": n_search, "n_taps": n_patch,
+                   "form": args.form if within else "fp32 parity", "iterations": args.warmup + args.steps,
+                   "noise": "global, sigma lists from the GPU estimator, shared by both forms",
+                   "plain_fp32": {"value": round(v_plain, 3), "ms_per_step": round(wall_p / args.steps * 1e3, 3)},
+                   "accelerated": {"form": args.form, "value": round(v_fast, 3),
+                                   "ms_per_step": round(wall_f / args.steps * 1e3, 3),
+                                   "speedup_vs_plain": round(v_fast / v_plain, 3),
+                                   "psnr_plain_db": round(psnr(mse_p), 4), "psnr_db": round(psnr(mse_f), 4),
+                                   "psnr_loss_db": round(d_psnr, 4), "rmse_vs_plain": round(rmse_fp, 4),
+                                   "maxabs_vs_plain": round(maxabs_fp, 4), "within_budget": bool(within),
+                                   "offset_pruning": "none (full search)"}},
+        "roofline": roof(fast_kernel if within else plain_kernel, kms_f if within else kms_p),
+        "roofline_plain_fp32": roof(plain_kernel, kms_p),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
