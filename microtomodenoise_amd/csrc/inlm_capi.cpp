@@ -368,7 +368,8 @@ inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
     }
     HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
     if (h->fast)
-        HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->raw, h->result,
+        HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems,
+                                            h->raw, h->result,
                                             h->sigma, h->p.beta, (float)(1.0 / (h->scale * h->scale)), h->shift,
                                             1.0 / h->scale, h->search_dev, (int)h->search.size(), h->taps, h->fg,
                                             h->stream));

@@ -65,10 +65,12 @@ hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice
 // volumes in the form's storage type; raw is the unpadded original in float
 // (returned unchanged where every weight vanishes).
 bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n);
-hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, const float *raw,
-                              float *out, const float *sigma, float beta, float mscale, double shift,
-                              double inv_scale, const SearchEntry *search, int n_search, const TapTable &taps,
-                              const FilterGeom &g, hipStream_t st);
+// elems: allocated elements of each padded volume from Rs/Ps on (bounds the
+// buffer loads).
+hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
+                              const float *raw, float *out, const float *sigma, float beta, float mscale,
+                              double shift, double inv_scale, const SearchEntry *search, int n_search,
+                              const TapTable &taps, const FilterGeom &g, hipStream_t st);
 hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
                                 hipStream_t st);
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);

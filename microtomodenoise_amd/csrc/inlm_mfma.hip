@@ -118,6 +118,8 @@ template <class TS>
 struct MfmaArgs {
     const TS *R;
     const TS *P;
+    const TS *R_end;   // one past the allocations of R and P (buffer ranges)
+    const TS *P_end;
     const float *raw;  // unpadded original
     float *out;
     const float *sigma;
@@ -162,6 +164,27 @@ __device__ __forceinline__ float fast_weight(float a)
     }
 }
 
+// Raw buffer loads: a uniform 128-bit resource (base, byte range) plus a
+// 32-bit per-lane byte offset.  Reads past the range return zero, so a load
+// can never leave the allocation the resource describes.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data format
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes)
+{
+    const int n = bytes <= 0 ? 0 : (bytes > 0x7fffffffL ? 0x7fffffff : (int)bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
+}
+template <class TS>
+__device__ __forceinline__ float buf_load1(rsrc_t r, unsigned off)
+{
+    if constexpr (sizeof(TS) == 4) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else {
+        const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        return (float)__builtin_bit_cast(TS, u);
+    }
+}
+
 template <class TS>
 __device__ __forceinline__ void load4(const TS *p, float (&v)[4])
 {
@@ -177,8 +200,78 @@ __device__ __forceinline__ void load4(const TS *p, float (&v)[4])
     }
 }
 
+// Staging of D = (P(y+s) - P(y))^2 into the 16-bit D plane, 4 columns per
+// chunk.  Generic form: FP32 arithmetic, clamped below the FP16 maximum when
+// the plane is FP16 (an infinite D would turn the zero band weights of its
+// neighbours into NaN).
+template <int R, class TS, class TD>
+struct Stager {
+    static constexpr int CPT = MTile<R>::CPT;
+    typedef float chunk[4];
+    chunk p0[CPT];
+    __device__ __forceinline__ void load(chunk &c, const TS *p) const { load4(p, c); }
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off) const
+    {
+        // the builtins return GCC-style vectors; copy them out whole (element
+        // subscripts on them are not lowered correctly by this compiler)
+        if constexpr (sizeof(TS) == 4) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+            __builtin_memcpy(c, &q, 16);
+        } else {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+            TS h[4];
+            __builtin_memcpy(h, &q, 8);
+#pragma unroll
+            for (int e = 0; e < 4; e++) c[e] = (float)h[e];
+        }
+    }
+    __device__ __forceinline__ void store(const chunk &pn, int c, TD *dst) const
+    {
+        TD dv[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const float df = pn[e] - p0[c][e];
+            float dd = df * df;
+            if constexpr (std::is_same<TD, _Float16>::value) dd = __builtin_fminf(dd, 65504.f);
+            dv[e] = DPlane<TD>::from(dd);
+        }
+        __builtin_memcpy(dst, dv, 8);
+    }
+};
+
+// FP16 storage and FP16 plane: packed half arithmetic, two values per
+// instruction (difference, square, clamp).
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+template <int R>
+struct Stager<R, _Float16, _Float16> {
+    static constexpr int CPT = MTile<R>::CPT;
+    typedef half2v chunk[2];
+    chunk p0[CPT];
+    __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, 8); }
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off) const
+    {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        __builtin_memcpy(&c[0], &q, 8);
+    }
+    __device__ __forceinline__ void store(const chunk &pn, int c, _Float16 *dst) const
+    {
+        const half2v big = {(_Float16)65504.f, (_Float16)65504.f};
+        half2v d0 = pn[0] - p0[c][0], d1 = pn[1] - p0[c][1];
+        d0 = __builtin_elementwise_min(d0 * d0, big);
+        d1 = __builtin_elementwise_min(d1 * d1, big);
+        half2v out[2] = {d0, d1};
+        __builtin_memcpy(dst, out, 8);
+    }
+};
+
+// 3 workgroups of 8 waves per CU (6 waves per SIMD) fit the LDS for R <= 2;
+// the register budget is capped to match.
+template <int R>
+constexpr int mfma_waves_per_eu() { return R <= 2 ? 6 : 4; }
+
 template <int R, class TS, class TD, int WFN>
-__global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
+__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R>(), mfma_waves_per_eu<R>())))
+void filter_mfma(MfmaArgs<TS> a)
 {
     using T = MTile<R>;
     using DP = DPlane<TD>;
@@ -218,8 +311,9 @@ __global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
     // Staging chunks: 4 columns of one region row.  Region row (rz, ry) is
     // slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
     const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
+    using ST = Stager<R, TS, TD>;
+    ST stg;
     int coff[T::CPT], loff[T::CPT];
-    float p0[T::CPT][4];
 #pragma unroll
     for (int c = 0; c < T::CPT; c++) {
         int ci = t + c * NTM;
@@ -228,7 +322,7 @@ __global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
         const int ry = row % T::RY, rz = row / T::RY;
         coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
         loff[c] = row * XW + 4 + 4 * cx;
-        load4(Pt + coff[c], p0[c]);
+        stg.load(stg.p0[c], Pt + coff[c]);
     }
 
     // This lane's outputs: C rows m = 4q + i  ->  (z pair member q/2, y 4(q%2)+i).
@@ -236,8 +330,12 @@ __global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
     const int x = X0 + 16 * xh + n;
     const int z = Z0 + 2 * gz + (q >> 1);
     const int yb = Y0 + 4 * (q & 1);
-    const long vox = (long)(z + g.pz) * sp + (long)(yb + g.py) * g.nxp + (x + g.px);
-    const TS *Rv = a.R + vox;
+    // R is read through the tile origin (uniform) plus a 32-bit byte offset
+    // per lane, so the loads take the scalar-base addressing form.
+    const TS *Rt = a.R + ((long)(Z0 + g.pz) * sp + (long)(Y0 + g.py) * g.nxp + (X0 + g.px));
+    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(yb - Y0) * g.nxp + (x - X0)) * sizeof(TS));
+    const unsigned rowbyte = (unsigned)(g.nxp * sizeof(TS));
+    const TS *Rv = reinterpret_cast<const TS *>(reinterpret_cast<const char *>(Rt) + rbyte);
     const int zc = z < g.nz ? z : g.nz - 1;
     const float sig = a.sigma[zc];
     const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
@@ -248,28 +346,22 @@ __global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
 
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f}, wmax[4] = {0.f, 0.f, 0.f, 0.f};
 
-    auto load_chunks = [&](float (&pn)[T::CPT][4], int lin) __attribute__((always_inline)) {
+    typedef typename ST::chunk chunk;
+    auto load_chunks = [&](chunk (&pn)[T::CPT], int lin) __attribute__((always_inline)) {
+        const TS *base = Pt + lin;  // uniform
+        const rsrc_t r = make_rsrc(base, (long)(a.P_end - base) * (long)sizeof(TS));
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++) load4(Pt + lin + coff[c], pn[c]);
+        for (int c = 0; c < T::CPT; c++) stg.load(pn[c], r, (unsigned)(coff[c] * (int)sizeof(TS)));
     };
-    auto stage = [&](const float (&pn)[T::CPT][4], TD *buf) __attribute__((always_inline)) {
+    auto stage = [&](const chunk (&pn)[T::CPT], TD *buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++) {
-            if (T::NCHUNK % NTM == 0 || t + c * NTM < T::NCHUNK) {
-                TD dv[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const float df = pn[c][e] - p0[c][e];
-                    dv[e] = DP::from(__builtin_fminf(df * df, 60000.f));
-                }
-                __builtin_memcpy(buf + loff[c], dv, 8);
-            }
-        }
+        for (int c = 0; c < T::CPT; c++)
+            if (T::NCHUNK % NTM == 0 || t + c * NTM < T::NCHUNK) stg.store(pn[c], c, buf + loff[c]);
     };
 
     __syncthreads();  // zeroed LDS visible before the first staging
     {
-        float pn[T::CPT][4];
+        chunk pn[T::CPT];
         load_chunks(pn, a.search[0].lin);
         stage(pn, lds[0]);
     }
@@ -279,11 +371,15 @@ __global__ __launch_bounds__(NTM) void filter_mfma(MfmaArgs<TS> a)
     for (int k = 0; k < nS; k++) {
         const int lin = a.search[k].lin;
         const bool more = (k + 1) < nS;
-        float pn[T::CPT][4];
+        chunk pn[T::CPT];
         if (more) load_chunks(pn, a.search[k + 1].lin);
         float rv[4];
+        {
+            const TS *rb = Rt + lin;  // uniform
+            const rsrc_t r = make_rsrc(rb, (long)(a.R_end - rb) * (long)sizeof(TS));
 #pragma unroll
-        for (int i = 0; i < 4; i++) rv[i] = (float)Rv[(long)i * g.nxp + lin];
+            for (int i = 0; i < 4; i++) rv[i] = buf_load1<TS>(r, rbyte + (unsigned)i * rowbyte);
+        }
 
         const TD *L = lds[k & 1] + aoff;
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -400,13 +496,16 @@ bool mfma_bands_r(const int *dxyz, const float *w, int ntap, float (*band)[8])
 }
 
 template <int R, class TS, class TD, int WFN>
-hipError_t launch_mfma_t(const void *Rs, const void *Ps, const float *raw, float *out, const float *sigma, float beta,
-                         float mscale, double shift, double inv_scale, const SearchEntry *search, int n_search,
-                         const TapTable &taps, const FilterGeom &g, hipStream_t st)
+hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
+                         const float *sigma, float beta, float mscale, double shift, double inv_scale,
+                         const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
+                         hipStream_t st)
 {
     MfmaArgs<TS> a;
     a.R = static_cast<const TS *>(Rs);
     a.P = static_cast<const TS *>(Ps);
+    a.R_end = a.R + elems;
+    a.P_end = a.P + elems;
     a.raw = raw;
     a.out = out;
     a.sigma = sigma;
@@ -427,21 +526,21 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, const float *raw, float
 }
 
 template <int R, int WFN>
-hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, const float *raw, float *out,
+hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
                             const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
                             hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
-        return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
-                                                      search, n_search, taps, g, st);
+        return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+                                                      inv_scale, search, n_search, taps, g, st);
     case FORM_FP16:
-        return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
-                                                         search, n_search, taps, g, st);
+        return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+                                                         inv_scale, search, n_search, taps, g, st);
     case FORM_BF16:
-        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale,
-                                                     search, n_search, taps, g, st);
+        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+                                                     inv_scale, search, n_search, taps, g, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -459,15 +558,15 @@ bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
     }
 }
 
-hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, const float *raw,
-                              float *out, const float *sigma, float beta, float mscale, double shift,
-                              double inv_scale, const SearchEntry *search, int n_search, const TapTable &taps,
-                              const FilterGeom &g, hipStream_t st)
+hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
+                              const float *raw, float *out, const float *sigma, float beta, float mscale,
+                              double shift, double inv_scale, const SearchEntry *search, int n_search,
+                              const TapTable &taps, const FilterGeom &g, hipStream_t st)
 {
 #define INLM_MF(RR)                                                                                               \
-    return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale, \
+    return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
                                                     search, n_search, taps, g, st)                                 \
-                          : launch_mfma_form<RR, 1>(form, Rs, Ps, raw, out, sigma, beta, mscale, shift, inv_scale, \
+                          : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
                                                     search, n_search, taps, g, st)
     switch (rp) {
     case 1: INLM_MF(1);
