@@ -186,6 +186,69 @@ INLM_API float inlm_last_filter_ms(const inlm_handle *h);
 /* Table sizes of a handle: offsets visited and patch taps. */
 INLM_API inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n_patch);
 
+/* ---------------------------------------------------------------------------
+ * Learned offset proposal.  None of these has a counterpart in the reference:
+ * they serve the pruned matrix-core path, in which each 32x8x8 filter tile
+ * (tiles numbered x fastest, then y, then z) visits only a subset of the
+ * search offsets.
+ * ------------------------------------------------------------------------- */
+typedef struct inlm_proposal inlm_proposal;
+
+/* Number of 32x8x8 filter tiles of a handle's volume. */
+INLM_API int64_t inlm_tile_count(const inlm_handle *h);
+
+/* Training labels: runs iteration `iter` exactly as inlm_run_iteration does
+ * with the FP32 parity form (same result, bit for bit) and also writes, for
+ * every tile and search offset, the sum over the tile's voxels of the weight
+ * that offset received ([tiles][n_search] floats, to device memory when
+ * is_device != 0).  FP32 handles whose patch runs on the tiled kernel only. */
+INLM_API inlm_status inlm_weight_mass(inlm_handle *h, int32_t iter, float *mass, int32_t is_device);
+
+/* A 3D CNN (four 3x3x3 convolutions of 16 channels, ReLU, per-tile mean
+ * pool, linear head over the n_search offsets, softmax) that predicts each
+ * tile's distribution of weight mass over the offsets.  Its convolutions run
+ * on BF16 matrix cores with FP32 accumulation. */
+INLM_API inlm_status inlm_proposal_create(int32_t n_search, int32_t device, uint64_t seed, inlm_proposal **out);
+INLM_API inlm_status inlm_proposal_destroy(inlm_proposal *p);
+INLM_API const char *inlm_proposal_last_error(const inlm_proposal *p);
+INLM_API int64_t inlm_proposal_num_params(const inlm_proposal *p);
+/* Parameter vector (host floats): w1[27][16] b1[16]; three times
+ * w[27 tap][16 in][16 out] b[16]; Wh[16][n_search] bh[n_search].  Taps are
+ * numbered (dz+1)*9 + (dy+1)*3 + (dx+1).  set != 0 loads `params` (and resets
+ * the optimiser state), set == 0 reads them out. */
+INLM_API inlm_status inlm_proposal_params(inlm_proposal *p, float *params, int32_t set);
+/* Gradient of the last training step's loss (host floats, same layout). */
+INLM_API inlm_status inlm_proposal_grads(const inlm_proposal *p, float *grads);
+/* One training step on a device volume (nx*ny*nz floats) and its device
+ * labels from inlm_weight_mass: forward, cross-entropy of the predicted
+ * distribution against mass/sum(mass) averaged over tiles, backward; with
+ * lr > 0 also one Adam update.  *loss receives the loss before the update. */
+INLM_API inlm_status inlm_proposal_train_step(inlm_proposal *p, const float *vol, const int32_t shape[3],
+                                              const float *mass, float lr, float *loss);
+/* Predicted distributions [tiles][n_search] for a device volume, written to
+ * device memory when is_device != 0, else to host memory. */
+INLM_API inlm_status inlm_proposal_predict(inlm_proposal *p, const float *vol, const int32_t shape[3], float *probs,
+                                           int32_t is_device);
+
+/* Attaches a proposal to a matrix-core handle whose original is loaded: the
+ * proposal predicts once from the original; every later iteration visits,
+ * per tile, its K most probable offsets in search order, except in tiles
+ * whose K offsets carry less than `threshold` of the predicted mass
+ * (fallback) and in a fraction `audit_frac` of tiles drawn afresh each
+ * iteration (audit), which visit the whole search.  p == NULL detaches (and
+ * clears fixed lists). */
+INLM_API inlm_status inlm_set_proposal(inlm_handle *h, inlm_proposal *p, int32_t K, float threshold,
+                                       float audit_frac, uint64_t seed);
+/* Fixed per-tile offset lists for a matrix-core handle (host arrays):
+ * lists[t*K + j] are indices into the search table, counts[t] in [1, n_search]
+ * entries of tile t are used, counts[t] == n_search means the whole search.
+ * Replaces an attached proposal. */
+INLM_API inlm_status inlm_set_offset_lists(inlm_handle *h, const int32_t *lists, const int32_t *counts, int32_t K);
+/* Statistics of the last pruned iteration: tiles, fallback tiles, audited
+ * tiles, mean predicted mass of the kept offsets, mean offsets visited per
+ * tile. */
+INLM_API inlm_status inlm_proposal_stats(const inlm_handle *h, float out[5]);
+
 #ifdef __cplusplus
 }
 #endif

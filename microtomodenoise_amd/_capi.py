@@ -27,6 +27,11 @@ EXPORTS = [
     "inlm_set_sigma", "inlm_load_raw", "inlm_run_iteration", "inlm_get_result", "inlm_draw_samples",
     "inlm_patchstd", "inlm_estimate_noise", "inlm_denoise", "inlm_tables", "inlm_selftest_division",
     "inlm_last_filter_ms", "inlm_table_sizes",
+    # learned offset proposal
+    "inlm_tile_count", "inlm_weight_mass", "inlm_proposal_create", "inlm_proposal_destroy",
+    "inlm_proposal_last_error", "inlm_proposal_num_params", "inlm_proposal_params", "inlm_proposal_grads",
+    "inlm_proposal_train_step", "inlm_proposal_predict", "inlm_set_proposal", "inlm_set_offset_lists",
+    "inlm_proposal_stats",
 ]
 
 
@@ -95,6 +100,22 @@ def lib():
         L.inlm_tables.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, i32p, i32p, i32p, i32p, f32p]
         L.inlm_selftest_division.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                              C.POINTER(C.c_uint32)]
+        L.inlm_tile_count.argtypes = [H]
+        L.inlm_tile_count.restype = C.c_int64
+        L.inlm_weight_mass.argtypes = [H, C.c_int32, C.c_void_p, C.c_int32]
+        L.inlm_proposal_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(H)]
+        L.inlm_proposal_destroy.argtypes = [H]
+        L.inlm_proposal_last_error.argtypes = [H]
+        L.inlm_proposal_last_error.restype = C.c_char_p
+        L.inlm_proposal_num_params.argtypes = [H]
+        L.inlm_proposal_num_params.restype = C.c_int64
+        L.inlm_proposal_params.argtypes = [H, f32p, C.c_int32]
+        L.inlm_proposal_grads.argtypes = [H, f32p]
+        L.inlm_proposal_train_step.argtypes = [H, C.c_void_p, i32p, C.c_void_p, C.c_float, f32p]
+        L.inlm_proposal_predict.argtypes = [H, C.c_void_p, i32p, C.c_void_p, C.c_int32]
+        L.inlm_set_proposal.argtypes = [H, H, C.c_int32, C.c_float, C.c_float, C.c_uint64]
+        L.inlm_set_offset_lists.argtypes = [H, i32p, i32p, C.c_int32]
+        L.inlm_proposal_stats.argtypes = [H, f32p]
         _LIB = L
     return _LIB
 
@@ -106,6 +127,12 @@ def ptr(a: np.ndarray, ctype):
 def check(status: int, handle=None):
     if status != INLM_OK:
         msg = lib().inlm_last_error(handle)
+        raise InlmError(status, msg.decode() if msg else "")
+
+
+def check_proposal(status: int, prop=None):
+    if status != INLM_OK:
+        msg = lib().inlm_proposal_last_error(prop)
         raise InlmError(status, msg.decode() if msg else "")
 
 

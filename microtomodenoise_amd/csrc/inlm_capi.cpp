@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "inlm_kernels.hpp"
+#include "inlm_learn.hpp"
 #include "inlm_tables.hpp"
 
 namespace {
@@ -50,6 +51,26 @@ struct inlm_handle {
     bool raw_loaded = false, sigma_set = false;
     int last_iter = 0;
     float last_ms = 0.f;
+    // pruned matrix-core path: per-tile offset lists, either fixed by the
+    // caller or re-drawn each iteration from an attached proposal's
+    // prediction (probs_dev)
+    int *lists_dev = nullptr, *counts_dev = nullptr;
+    int lists_K = 0;
+    long lists_cap = 0;
+    bool lists_fixed = false;
+    float *probs_dev = nullptr;
+    int prop_K = 0;
+    float prop_thr = 0.f, prop_audit = 0.f;
+    uint64_t prop_seed = 0;
+    float *stats_dev = nullptr;
+    float stats[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    std::string err;
+};
+
+struct inlm_proposal {
+    inlm::ProposalNet net;
+    int device = 0;
+    int ns = 0;
     std::string err;
 };
 
@@ -89,6 +110,36 @@ size_t padded_alloc_elems(const inlm::Geometry &g)
     long maxidx = zmax * sp + ymax * g.nxp + xmax + 2 * R;
     long need = std::max<long>(maxidx + 1, (long)g.voxels_p());
     return (size_t)need + 64;
+}
+
+long tile_count(const inlm::Geometry &g)
+{
+    return (long)((g.nx + 31) / 32) * ((g.ny + 7) / 8) * ((g.nz + 7) / 8);
+}
+
+inlm_status ensure_lists(inlm_handle *h, int K)
+{
+    const long nt = tile_count(h->geo);
+    const long need = nt * (long)K;
+    if (!h->counts_dev) HIP_TRY(h, hipMalloc(&h->counts_dev, sizeof(int) * nt));
+    if (!h->stats_dev) HIP_TRY(h, hipMalloc(&h->stats_dev, sizeof(float) * 4));
+    if (need > h->lists_cap) {
+        (void)hipFree(h->lists_dev);
+        h->lists_dev = nullptr;
+        h->lists_cap = 0;
+        HIP_TRY(h, hipMalloc(&h->lists_dev, sizeof(int) * need));
+        h->lists_cap = need;
+    }
+    h->lists_K = K;
+    return INLM_OK;
+}
+
+void clear_pruning(inlm_handle *h)
+{
+    (void)hipFree(h->probs_dev);
+    h->probs_dev = nullptr;
+    h->lists_fixed = false;
+    h->prop_K = 0;
 }
 
 inlm_status ensure_samples(inlm_handle *h, int64_t n)
@@ -294,6 +345,10 @@ inlm_status inlm_destroy(inlm_handle *h)
     (void)hipFree(h->search_dev);
     (void)hipFree(h->samples_dev);
     (void)hipFree(h->std_dev);
+    (void)hipFree(h->lists_dev);
+    (void)hipFree(h->counts_dev);
+    (void)hipFree(h->probs_dev);
+    (void)hipFree(h->stats_dev);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -342,10 +397,18 @@ inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device)
     }
     h->raw_loaded = true;
     h->last_iter = 0;
+    // a proposal predicted from the previous original no longer applies
+    if (h->probs_dev) clear_pruning(h);
     return INLM_OK;
 }
 
-inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
+}  // extern "C"
+
+namespace {
+
+// One iteration (inlm_run_iteration); with `mass` (device) the FP32 tiled
+// kernel also writes the per-tile weight mass of every offset.
+inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, float *mass)
 {
     if (!h) return fail(h, INLM_BAD_ARG, "null handle");
     if (iter < 1) return fail(h, INLM_BAD_ARG, "iterations count from 1");
@@ -366,22 +429,51 @@ inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
             HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->P_pad, h->fg, h->stream));
         P = h->Ps;
     }
+    const int ns = (int)h->search.size();
+    const long nt = tile_count(h->geo);
     HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
-    if (h->fast)
+    if (h->fast) {
+        inlm::OffsetLists ol{h->lists_dev, h->counts_dev, h->lists_K, nt};
+        const bool pruned = h->probs_dev || h->lists_fixed;
+        if (h->probs_dev) {
+            // fresh lists for this iteration: top-K, fallback, audit draw
+            HIP_TRY(h, hipMemsetAsync(h->stats_dev, 0, sizeof(float) * 4, h->stream));
+            HIP_TRY(h, inlm::launch_topk(h->probs_dev, (int)nt, ns, h->prop_K, h->prop_thr, h->prop_audit,
+                                         h->prop_seed + 0x9E3779B97F4A7C15ull * (uint64_t)iter, h->lists_dev,
+                                         h->counts_dev, h->stats_dev, h->stream));
+        }
         HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems,
-                                            h->raw, h->result,
-                                            h->sigma, h->p.beta, (float)(1.0 / (h->scale * h->scale)), h->shift,
-                                            1.0 / h->scale, h->search_dev, (int)h->search.size(), h->taps, h->fg,
-                                            h->stream));
-    else
+                                            h->raw, h->result, h->sigma, h->p.beta,
+                                            (float)(1.0 / (h->scale * h->scale)), h->shift, 1.0 / h->scale,
+                                            h->search_dev, ns, h->taps, h->fg, h->stream, pruned ? &ol : nullptr));
+    } else {
         HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad,
                                        static_cast<const float *>(P), h->result, h->sigma, h->p.beta, h->search_dev,
-                                       (int)h->search.size(), h->taps, h->fg, h->stream));
+                                       ns, h->taps, h->fg, h->stream, mass));
+    }
     HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     HIP_TRY(h, hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+    if (h->fast && h->probs_dev) {
+        float st[4];
+        HIP_TRY(h, hipMemcpy(st, h->stats_dev, sizeof st, hipMemcpyDeviceToHost));
+        h->stats[0] = (float)nt;
+        h->stats[1] = st[0];
+        h->stats[2] = st[1];
+        h->stats[3] = st[2] / (float)nt;
+        h->stats[4] = st[3] / (float)nt;
+    }
     h->last_iter = iter;
     return INLM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+inlm_status inlm_run_iteration(inlm_handle *h, int32_t iter, const float *raw)
+{
+    return run_iteration_impl(h, iter, raw, nullptr);
 }
 
 inlm_status inlm_get_result(inlm_handle *h, float *out, int32_t is_device)
@@ -579,6 +671,188 @@ inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n
     if (!h) return INLM_BAD_ARG;
     if (n_search) *n_search = (int32_t)h->search.size();
     if (n_patch) *n_patch = h->taps.n;
+    return INLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned offset proposal
+// ---------------------------------------------------------------------------
+
+int64_t inlm_tile_count(const inlm_handle *h) { return h ? (int64_t)tile_count(h->geo) : 0; }
+
+inlm_status inlm_weight_mass(inlm_handle *h, int32_t iter, float *mass, int32_t is_device)
+{
+    if (!h || !mass) return fail(h, INLM_BAD_ARG, "null argument");
+    if (h->fast || h->rp_tiled == 0)
+        return fail(h, INLM_BAD_ARG, "weight mass needs the FP32 form on the tiled kernel (isotropic patch r 1..3)");
+    HIP_TRY(h, hipSetDevice(h->device));
+    const size_t bytes = sizeof(float) * (size_t)tile_count(h->geo) * h->search.size();
+    float *dst = mass;
+    if (!is_device) HIP_TRY(h, hipMalloc(&dst, bytes));
+    inlm_status s = run_iteration_impl(h, iter, nullptr, dst);
+    if (s == INLM_OK && !is_device) {
+        hipError_t e = hipMemcpy(mass, dst, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) s = hip_fail(h, e, "hipMemcpy(mass)");
+    }
+    if (!is_device) (void)hipFree(dst);
+    return s;
+}
+
+namespace {
+inlm_status pfail(inlm_proposal *p, inlm_status st, const std::string &msg)
+{
+    if (p) p->err = msg;
+    else g_create_error = msg;
+    return st;
+}
+inlm_status phip(inlm_proposal *p, hipError_t e, const char *where)
+{
+    return pfail(p, e == hipErrorOutOfMemory ? INLM_OOM : INLM_HIP_ERROR, std::string(where) + ": " +
+                                                                              hipGetErrorString(e));
+}
+}  // namespace
+
+inlm_status inlm_proposal_create(int32_t n_search, int32_t device, uint64_t seed, inlm_proposal **out)
+{
+    if (!out) return pfail(nullptr, INLM_BAD_ARG, "null argument");
+    *out = nullptr;
+    if (n_search < 1 || n_search > inlm::kMaxProposalOffsets)
+        return pfail(nullptr, INLM_BAD_ARG, "n_search must be in [1, 2048]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return pfail(nullptr, INLM_NO_DEVICE, "no GPU visible");
+    if (device < 0 || device >= ndev) return pfail(nullptr, INLM_BAD_ARG, "device index out of range");
+    inlm_proposal *p = new inlm_proposal();
+    p->device = device;
+    p->ns = n_search;
+    hipError_t e = p->net.init(n_search, device, seed);
+    if (e != hipSuccess) {
+        inlm_status st = phip(nullptr, e, "proposal init");
+        delete p;
+        return st;
+    }
+    *out = p;
+    return INLM_OK;
+}
+
+inlm_status inlm_proposal_destroy(inlm_proposal *p)
+{
+    delete p;
+    return INLM_OK;
+}
+
+const char *inlm_proposal_last_error(const inlm_proposal *p) { return p ? p->err.c_str() : g_create_error.c_str(); }
+
+int64_t inlm_proposal_num_params(const inlm_proposal *p) { return p ? (int64_t)p->net.num_params() : 0; }
+
+inlm_status inlm_proposal_params(inlm_proposal *p, float *params, int32_t set)
+{
+    if (!p || !params) return pfail(p, INLM_BAD_ARG, "null argument");
+    hipError_t e = set ? p->net.set_params(params, p->net.num_params()) : p->net.get_params(params, p->net.num_params());
+    return e == hipSuccess ? INLM_OK : phip(p, e, "proposal params");
+}
+
+inlm_status inlm_proposal_grads(const inlm_proposal *p, float *grads)
+{
+    if (!p || !grads) return pfail(const_cast<inlm_proposal *>(p), INLM_BAD_ARG, "null argument");
+    hipError_t e = p->net.get_grads(grads, p->net.num_params());
+    return e == hipSuccess ? INLM_OK : phip(const_cast<inlm_proposal *>(p), e, "proposal grads");
+}
+
+inlm_status inlm_proposal_train_step(inlm_proposal *p, const float *vol, const int32_t shape[3], const float *mass,
+                                     float lr, float *loss)
+{
+    if (!p || !vol || !shape || !mass) return pfail(p, INLM_BAD_ARG, "null argument");
+    if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1) return pfail(p, INLM_BAD_ARG, "shape must be positive");
+    const int sh[3] = {shape[0], shape[1], shape[2]};
+    float l = 0.f;
+    hipError_t e = p->net.run(vol, sh, mass, true, &l);
+    if (e != hipSuccess) return phip(p, e, "proposal forward/backward");
+    if (loss) *loss = l;
+    if (lr > 0.f && (e = p->net.adam(lr)) != hipSuccess) return phip(p, e, "proposal update");
+    return INLM_OK;
+}
+
+inlm_status inlm_proposal_predict(inlm_proposal *p, const float *vol, const int32_t shape[3], float *probs,
+                                  int32_t is_device)
+{
+    if (!p || !vol || !shape || !probs) return pfail(p, INLM_BAD_ARG, "null argument");
+    if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1) return pfail(p, INLM_BAD_ARG, "shape must be positive");
+    const int sh[3] = {shape[0], shape[1], shape[2]};
+    hipError_t e = p->net.run(vol, sh, nullptr, false, nullptr);
+    if (e != hipSuccess) return phip(p, e, "proposal forward");
+    const long nt = (long)((sh[0] + 31) / 32) * ((sh[1] + 7) / 8) * ((sh[2] + 7) / 8);
+    e = hipMemcpy(probs, p->net.probs(), sizeof(float) * nt * p->ns,
+                  is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost);
+    return e == hipSuccess ? INLM_OK : phip(p, e, "hipMemcpy(probs)");
+}
+
+inlm_status inlm_set_proposal(inlm_handle *h, inlm_proposal *p, int32_t K, float threshold, float audit_frac,
+                              uint64_t seed)
+{
+    if (!h) return fail(h, INLM_BAD_ARG, "null handle");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (!p) {
+        clear_pruning(h);
+        return INLM_OK;
+    }
+    if (!h->fast) return fail(h, INLM_BAD_ARG, "offset pruning needs a matrix-core form");
+    if (!h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
+    const int ns = (int)h->search.size();
+    if (p->ns != ns) return fail(h, INLM_BAD_ARG, "proposal was built for a different number of search offsets");
+    if (p->device != h->device) return fail(h, INLM_BAD_ARG, "proposal lives on another device");
+    if (K < 1 || K > ns) return fail(h, INLM_BAD_ARG, "K must be in [1, n_search]");
+    if (!(threshold >= 0.f && threshold <= 1.f) || !(audit_frac >= 0.f && audit_frac <= 1.f))
+        return fail(h, INLM_BAD_ARG, "threshold and audit_frac must lie in [0, 1]");
+    clear_pruning(h);
+    inlm_status s = ensure_lists(h, K);
+    if (s != INLM_OK) return s;
+    const long nt = tile_count(h->geo);
+    HIP_TRY(h, hipMalloc(&h->probs_dev, sizeof(float) * nt * ns));
+    const int sh[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
+    hipError_t e = p->net.run(h->raw, sh, nullptr, false, nullptr);
+    if (e != hipSuccess) {
+        clear_pruning(h);
+        return hip_fail(h, e, "proposal forward");
+    }
+    HIP_TRY(h, hipMemcpy(h->probs_dev, p->net.probs(), sizeof(float) * nt * ns, hipMemcpyDeviceToDevice));
+    h->prop_K = K;
+    h->prop_thr = threshold;
+    h->prop_audit = audit_frac;
+    h->prop_seed = seed;
+    return INLM_OK;
+}
+
+inlm_status inlm_set_offset_lists(inlm_handle *h, const int32_t *lists, const int32_t *counts, int32_t K)
+{
+    if (!h || !lists || !counts) return fail(h, INLM_BAD_ARG, "null argument");
+    if (!h->fast) return fail(h, INLM_BAD_ARG, "offset pruning needs a matrix-core form");
+    const int ns = (int)h->search.size();
+    if (K < 1 || K > ns) return fail(h, INLM_BAD_ARG, "K must be in [1, n_search]");
+    const long nt = tile_count(h->geo);
+    for (long t = 0; t < nt; t++) {
+        if (counts[t] < 1 || counts[t] > ns) return fail(h, INLM_BAD_ARG, "counts must lie in [1, n_search]");
+        const int used = counts[t] < ns ? counts[t] : 0;
+        if (used > K) return fail(h, INLM_BAD_ARG, "a count exceeds K");
+        for (int j = 0; j < used; j++)
+            if (lists[t * K + j] < 0 || lists[t * K + j] >= ns)
+                return fail(h, INLM_BAD_ARG, "list entry outside the search table");
+    }
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    clear_pruning(h);
+    inlm_status s = ensure_lists(h, K);
+    if (s != INLM_OK) return s;
+    HIP_TRY(h, hipMemcpy(h->lists_dev, lists, sizeof(int) * nt * K, hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(h->counts_dev, counts, sizeof(int) * nt, hipMemcpyHostToDevice));
+    h->lists_fixed = true;
+    return INLM_OK;
+}
+
+inlm_status inlm_proposal_stats(const inlm_handle *h, float out[5])
+{
+    if (!h || !out) return INLM_BAD_ARG;
+    for (int i = 0; i < 5; i++) out[i] = h->stats[i];
     return INLM_OK;
 }
 

@@ -133,6 +133,7 @@ struct TiledArgs {
     FilterGeom g;
     int ntx, nty, ntz;
     float wc[kMaxClasses];  // weight per class (class 0 = centre)
+    float *mass;            // optional: [tile][offset] sum of the weights over the tile's voxels
 };
 
 // Correctly rounded n/d for the operands the Pade weight produces
@@ -195,12 +196,13 @@ __device__ __forceinline__ int xcd_swizzle(int wg, int nwg)
 template <int R>
 constexpr int waves_per_eu() { return R <= 2 ? 6 : 4; }
 
-template <int R, int WFN>
+template <int R, int WFN, bool MASS>
 __global__ __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(waves_per_eu<R>(), waves_per_eu<R>()))) void
 filter_tiled_fp32(TiledArgs a)
 {
     using T = Tile<R>;
     __shared__ __attribute__((aligned(16))) float lds[2][T::LDS];
+    __shared__ float red[2][NTF / 64];
 
     const int t = threadIdx.x;
     const int wg = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -235,6 +237,7 @@ filter_tiled_fp32(TiledArgs a)
     const int zc = z < g.nz ? z : g.nz - 1;
     const float sig = a.sigma[zc];
     const float m = -1.f / (sig * sig * a.beta);
+    const bool valid = y < g.ny && z < g.nz;
 
     float fv[VX], fw[VX], wmax[VX];
 #pragma unroll
@@ -327,15 +330,32 @@ filter_tiled_fp32(TiledArgs a)
             });
             pin(d);
         });
+        float sw = 0.f;
 #pragma unroll
         for (int v = 0; v < VX; v++) {
             const float wt = weight_of<WFN>(d[v] * m);
             fv[v] = fv[v] + wt * rs[v];
             fw[v] = fw[v] + wt;
             wmax[v] = __builtin_fmaxf(wmax[v], wt);  // == (wt > wmax ? wt : wmax): weights are >= 0, never NaN
+            if constexpr (MASS) sw += (valid && x0 + v < g.nx) ? wt : 0.f;
+        }
+        if constexpr (MASS) {
+            // weight mass of this offset over the tile: wave sums into LDS,
+            // thread 0 adds the 8 wave sums after the barrier below
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) sw += __shfl_xor(sw, o, 64);
+            if ((t & 63) == 0) red[k & 1][t >> 6] = sw;
         }
         if (more) stage(pn, lds[(k + 1) & 1]);
         __syncthreads();
+        if constexpr (MASS) {
+            if (t == 0) {
+                float tot = 0.f;
+#pragma unroll
+                for (int w = 0; w < NTF / 64; w++) tot += red[k & 1][w];
+                a.mass[(long)wg * nS + k] = tot;
+            }
+        }
     }
 
     if (y < g.ny && z < g.nz) {
@@ -558,9 +578,10 @@ hipError_t launch_pad_reflect(const float *in, float *out, const FilterGeom &g, 
 template <int R, int WFN>
 static hipError_t launch_tiled(const float *Rv, const float *P, float *out, const float *sigma, float beta,
                                const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
-                               hipStream_t st)
+                               float *mass, hipStream_t st)
 {
     TiledArgs a;
+    a.mass = mass;
     a.R = Rv;
     a.P = P;
     a.out = out;
@@ -575,7 +596,10 @@ static hipError_t launch_tiled(const float *Rv, const float *P, float *out, cons
     for (int i = 0; i < kMaxClasses; i++) a.wc[i] = 0.f;
     if (!class_weights_r<R>(taps.dxyz, taps.w, taps.n, a.wc)) return hipErrorInvalidValue;
     const long nwg = (long)a.ntx * a.nty * a.ntz;
-    hipLaunchKernelGGL((filter_tiled_fp32<R, WFN>), dim3((unsigned)nwg), dim3(NTF), 0, st, a);
+    if (mass)
+        hipLaunchKernelGGL((filter_tiled_fp32<R, WFN, true>), dim3((unsigned)nwg), dim3(NTF), 0, st, a);
+    else
+        hipLaunchKernelGGL((filter_tiled_fp32<R, WFN, false>), dim3((unsigned)nwg), dim3(NTF), 0, st, a);
     return hipGetLastError();
 }
 
@@ -601,12 +625,12 @@ static hipError_t launch_pervoxel(const float *Rv, const float *P, float *out, c
 
 hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const float *P, float *out,
                          const float *sigma, float beta, const SearchEntry *search, int n_search,
-                         const TapTable &taps, const FilterGeom &g, hipStream_t st)
+                         const TapTable &taps, const FilterGeom &g, hipStream_t st, float *mass)
 {
     if (form != FORM_FP32) return hipErrorInvalidValue;
 #define INLM_TILED(RR)                                                                                   \
-    return weight_fn == 0 ? launch_tiled<RR, 0>(R, P, out, sigma, beta, search, n_search, taps, g, st)   \
-                          : launch_tiled<RR, 1>(R, P, out, sigma, beta, search, n_search, taps, g, st)
+    return weight_fn == 0 ? launch_tiled<RR, 0>(R, P, out, sigma, beta, search, n_search, taps, g, mass, st) \
+                          : launch_tiled<RR, 1>(R, P, out, sigma, beta, search, n_search, taps, g, mass, st)
     switch (rp) {
     case 1: INLM_TILED(1);
     case 2: INLM_TILED(2);
@@ -614,6 +638,7 @@ hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const 
     default: break;
     }
 #undef INLM_TILED
+    if (mass) return hipErrorInvalidValue;  // weight mass needs the tiled kernel
     return weight_fn == 0 ? launch_pervoxel<0>(R, P, out, sigma, beta, search, n_search, taps, g, st)
                           : launch_pervoxel<1>(R, P, out, sigma, beta, search, n_search, taps, g, st);
 }

@@ -38,10 +38,12 @@ hipError_t launch_pad_reflect(const float *in, float *out, const FilterGeom &g, 
 // One filter pass.  R/P are padded volumes; out is unpadded; sigma has nz
 // entries.  Returns hipErrorInvalidValue when no kernel covers the patch.
 // rp is the (isotropic) patch radius served by a tiled kernel, or 0 to force
-// the per-voxel kernel, which runs any tap table.
+// the per-voxel kernel, which runs any tap table.  With mass != nullptr the
+// tiled kernel also writes, per 32x8x8 tile and search offset, the sum of
+// the weights over the tile's voxels ([tile][offset], tiles x-fastest).
 hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const float *P, float *out,
                          const float *sigma, float beta, const SearchEntry *search, int n_search,
-                         const TapTable &taps, const FilterGeom &g, hipStream_t st);
+                         const TapTable &taps, const FilterGeom &g, hipStream_t st, float *mass = nullptr);
 
 // Whether the tiled kernel for isotropic radius rp uses exactly the given tap
 // order (dx,dy,dz per tap, centre first) and whether all taps it groups into
@@ -67,10 +69,20 @@ hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice
 bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n);
 // elems: allocated elements of each padded volume from Rs/Ps on (bounds the
 // buffer loads).
+// Per-tile offset lists for the matrix-core filter (32x8x8 tiles, x fastest):
+// tile t visits the search entries lists[t*K .. t*K + counts[t]) when
+// counts[t] < n_search, else the whole search.  Device pointers.
+struct OffsetLists {
+    const int *lists;
+    const int *counts;
+    int K;
+    long ntiles;
+};
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
                               const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
-                              const TapTable &taps, const FilterGeom &g, hipStream_t st);
+                              const TapTable &taps, const FilterGeom &g, hipStream_t st,
+                              const OffsetLists *ol = nullptr);
 hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
                                 hipStream_t st);
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);

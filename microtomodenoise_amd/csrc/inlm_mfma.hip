@@ -132,6 +132,11 @@ struct MfmaArgs {
     FilterGeom g;
     int ntx, nty, ntz;
     float band[kMfmaMaxPatterns][8];  // band[p][dx + R]: weight of pattern p at dx
+    // optional per-tile offset lists: tile t visits lists[t*K .. t*K+counts[t])
+    // (indices into search, ascending) when counts[t] < n_search, else all
+    const int *lists;
+    const int *counts;
+    int K;
 };
 
 template <int R>
@@ -269,7 +274,7 @@ struct Stager<R, _Float16, _Float16> {
 template <int R>
 constexpr int mfma_waves_per_eu() { return R <= 2 ? 6 : 4; }
 
-template <int R, class TS, class TD, int WFN>
+template <int R, class TS, class TD, int WFN, bool PRUNED>
 __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R>(), mfma_waves_per_eu<R>())))
 void filter_mfma(MfmaArgs<TS> a)
 {
@@ -359,20 +364,39 @@ void filter_mfma(MfmaArgs<TS> a)
             if (T::NCHUNK % NTM == 0 || t + c * NTM < T::NCHUNK) stg.store(pn[c], c, buf + loff[c]);
     };
 
+    // offsets this tile visits: the whole search, or its list
+    int nS = a.n_search;
+    const int *lst = nullptr;
+    if constexpr (PRUNED) {
+        const int cnt = a.counts[wg];
+        if (cnt >= 1 && cnt < nS) {
+            nS = cnt;
+            lst = a.lists + (long)wg * a.K;
+        }
+    }
+    auto lin_of = [&](int k) __attribute__((always_inline)) {
+        if constexpr (PRUNED) {
+            if (lst) {
+                const unsigned i = (unsigned)lst[k];
+                return a.search[i < (unsigned)a.n_search ? i : 0u].lin;
+            }
+        }
+        return a.search[k].lin;
+    };
+
     __syncthreads();  // zeroed LDS visible before the first staging
     {
         chunk pn[T::CPT];
-        load_chunks(pn, a.search[0].lin);
+        load_chunks(pn, lin_of(0));
         stage(pn, lds[0]);
     }
     __syncthreads();
 
-    const int nS = a.n_search;
     for (int k = 0; k < nS; k++) {
-        const int lin = a.search[k].lin;
+        const int lin = lin_of(k);
         const bool more = (k + 1) < nS;
         chunk pn[T::CPT];
-        if (more) load_chunks(pn, a.search[k + 1].lin);
+        if (more) load_chunks(pn, lin_of(k + 1));
         float rv[4];
         {
             const TS *rb = Rt + lin;  // uniform
@@ -499,9 +523,12 @@ template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
                          const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
-                         hipStream_t st)
+                         const OffsetLists *ol, hipStream_t st)
 {
     MfmaArgs<TS> a;
+    a.lists = ol ? ol->lists : nullptr;
+    a.counts = ol ? ol->counts : nullptr;
+    a.K = ol ? ol->K : 0;
     a.R = static_cast<const TS *>(Rs);
     a.P = static_cast<const TS *>(Ps);
     a.R_end = a.R + elems;
@@ -521,7 +548,11 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const flo
     a.ntz = (g.nz + MTZ - 1) / MTZ;
     if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
     const long nwg = (long)a.ntx * a.nty * a.ntz;
-    hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
+    if (ol && (ol->ntiles != nwg || !ol->lists || !ol->counts || ol->K < 1)) return hipErrorInvalidValue;
+    if (ol)
+        hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
+    else
+        hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, false>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
     return hipGetLastError();
 }
 
@@ -529,18 +560,18 @@ template <int R, int WFN>
 hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
                             const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
-                            hipStream_t st)
+                            const OffsetLists *ol, hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
         return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                      inv_scale, search, n_search, taps, g, st);
+                                                      inv_scale, search, n_search, taps, g, ol, st);
     case FORM_FP16:
         return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                         inv_scale, search, n_search, taps, g, st);
+                                                         inv_scale, search, n_search, taps, g, ol, st);
     case FORM_BF16:
         return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, n_search, taps, g, st);
+                                                     inv_scale, search, n_search, taps, g, ol, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -561,13 +592,13 @@ bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
                               const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
-                              const TapTable &taps, const FilterGeom &g, hipStream_t st)
+                              const TapTable &taps, const FilterGeom &g, hipStream_t st, const OffsetLists *ol)
 {
 #define INLM_MF(RR)                                                                                               \
     return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, taps, g, st)                                 \
+                                                    search, n_search, taps, g, ol, st)                                 \
                           : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, taps, g, st)
+                                                    search, n_search, taps, g, ol, st)
     switch (rp) {
     case 1: INLM_MF(1);
     case 2: INLM_MF(2);

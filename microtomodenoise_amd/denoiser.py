@@ -220,6 +220,40 @@ class IterativeNLM_GPU:
         self._check(capi.lib().inlm_table_sizes(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    # -- learned offset proposal (no counterpart in the reference) -------------
+    def tile_count(self) -> int:
+        return int(capi.lib().inlm_tile_count(self._h))
+
+    def weight_mass(self, iter: int, dev_ptr: Optional[int] = None):
+        """Runs iteration `iter` (FP32 form) and returns the per-tile weight
+        mass of every search offset ([tiles, n_search] floats), or writes it
+        to device memory at dev_ptr."""
+        if dev_ptr is not None:
+            self._check(capi.lib().inlm_weight_mass(self._h, iter, C.c_void_p(dev_ptr), 1))
+            return None
+        ns, _ = self.table_sizes()
+        out = np.empty((self.tile_count(), ns), np.float32)
+        self._check(capi.lib().inlm_weight_mass(self._h, iter, out.ctypes.data_as(C.c_void_p), 0))
+        return out
+
+    def set_proposal(self, proposal, K: int, threshold: float = 0.95, audit_frac: float = 0.02, seed: int = 0):
+        """Attach a trained OffsetProposal (None detaches)."""
+        ph = proposal.handle if proposal is not None else None
+        self._check(capi.lib().inlm_set_proposal(self._h, ph, int(K), float(threshold), float(audit_frac),
+                                                 int(seed)))
+
+    def set_offset_lists(self, lists, counts, K: int):
+        lists = np.ascontiguousarray(lists, np.int32)
+        counts = np.ascontiguousarray(counts, np.int32)
+        self._check(capi.lib().inlm_set_offset_lists(self._h, capi.ptr(lists, C.c_int32),
+                                                     capi.ptr(counts, C.c_int32), int(K)))
+
+    def proposal_stats(self) -> dict:
+        out = np.zeros(5, np.float32)
+        self._check(capi.lib().inlm_proposal_stats(self._h, capi.ptr(out, C.c_float)))
+        return {"tiles": int(out[0]), "fallback_tiles": int(out[1]), "audited_tiles": int(out[2]),
+                "mean_predicted_mass": float(out[3]), "mean_offsets_visited": float(out[4])}
+
 
 def denoise(instack: np.ndarray, params: DenoiseParameters, device: int = 0):
     """Full iterNLM on one GPU: returns (result, per-iteration sigma lists)."""

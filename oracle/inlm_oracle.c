@@ -314,6 +314,45 @@ ORC_API void orc_filter_f32(const float *Rpad, const float *Ppad, const int shap
     free(orig_all);
 }
 
+/* Weight mass per filter tile and search offset: the weights of
+ * orc_filter_f32 (same FP32 operations, so the same values), summed in
+ * double over the voxels of each tile of tile[0] x tile[1] x tile[2] voxels
+ * (tiles numbered x fastest, then y, then z).  mass has ntiles * nS entries
+ * and is overwritten.  The reference has no such output; it labels the
+ * learned offset proposal. */
+ORC_API void orc_weight_mass_f32(const float *Ppad, const int shape[3], const int pad[3], const float *sigma,
+                                 float beta, int nS, const long long *soff, int nP, const long long *poff,
+                                 const float *w, int weight_mode, const int tile[3], double *mass)
+{
+    const int nx = shape[0], ny = shape[1], nz = shape[2];
+    const long long nxp = nx + 2 * pad[0], nyp = ny + 2 * pad[1];
+    const long long nslp = nxp * nyp;
+    const int ntx = (nx + tile[0] - 1) / tile[0], nty = (ny + tile[1] - 1) / tile[1];
+    const int ntz = (nz + tile[2] - 1) / tile[2];
+    memset(mass, 0, sizeof(double) * (size_t)ntx * nty * ntz * nS);
+    for (int z = 0; z < nz; z++) {
+        const float s = sigma[z];
+        const float m = -1.f / (s * s * beta);
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const long long i0 = (z + pad[2]) * nslp + (y + pad[1]) * nxp + (x + pad[0]);
+                const long long t = ((long long)(z / tile[2]) * nty + y / tile[1]) * ntx + x / tile[0];
+                double *mt = mass + t * nS;
+                for (int k = 0; k < nS; k++) {
+                    const long long i1 = i0 + soff[k];
+                    float d = 0.f;
+                    for (int p = 0; p < nP; p++) {
+                        float dd = Ppad[i1 + poff[p]] - Ppad[i0 + poff[p]];
+                        float t2 = dd * dd;
+                        d = d + t2 * w[p];
+                    }
+                    d = d * m;
+                    mt[k] += (double)(weight_mode == 0 ? pade_cut_f32(d) : expf(d));
+                }
+            }
+    }
+}
+
 /* FP64 twin of orc_filter_f32: identical algorithm, every intermediate in
  * double (inputs and weights are the same float values). */
 ORC_API void orc_filter_f64(const float *Rpad, const float *Ppad, const int shape[3], const int pad[3],

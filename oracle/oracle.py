@@ -52,6 +52,8 @@ def orc():
                                      _i64p, _f32p, C.c_int, _f32p, C.c_int, C.c_int, C.c_int]
         L.orc_filter_f64.argtypes = [_f32p, _f32p, _i32p, _i32p, _f64p, C.c_double, C.c_int, _i64p, C.c_int,
                                      _i64p, _f32p, C.c_int, _f64p, C.c_int, C.c_int, C.c_int]
+        L.orc_weight_mass_f32.argtypes = [_f32p, _i32p, _i32p, _f32p, C.c_float, C.c_int, _i64p, C.c_int, _i64p,
+                                          _f32p, C.c_int, _i32p, _f64p]
         L.orc_requant_u16.argtypes = [_f32p, C.c_longlong, _u16p]
         L.orc_draw_samples.argtypes = [_i32p, C.c_longlong, C.c_int, C.c_uint64, _i64p]
         L.orc_patchstd_f32.argtypes = [_f32p, _i32p, _i64p, C.c_longlong, C.c_int, _f32p]
@@ -190,12 +192,14 @@ def pad_reflect(vol: np.ndarray, pad) -> np.ndarray:
 
 
 def filter_iteration(R: np.ndarray, P: np.ndarray, sigma, p: Params, *, precision="f32", weights=None,
-                     z_range=None, nthreads=0) -> np.ndarray:
+                     z_range=None, nthreads=0, offsets=None) -> np.ndarray:
     """One iterNLM filter pass of the restated CPU path.
 
     R is the original noisy volume, P the previous iterate (R at iteration
     1), sigma a per-slice list.  precision 'f32' is operation-faithful to the
     reference CPU path; 'f64' is the FP64 oracle of the same algorithm.
+    `offsets` (indices into the search table) restricts the pass to a subset
+    of the search, as the pruned filter visits it.
     """
     R = np.ascontiguousarray(R, np.float32)
     P = np.ascontiguousarray(P, np.float32)
@@ -206,6 +210,8 @@ def filter_iteration(R: np.ndarray, P: np.ndarray, sigma, p: Params, *, precisio
     S = search_offsets(p.radius_searchspace, p.nslices)
     PT = patch_offsets(p.radius_patchspace, p.nslices)
     soff, poff = linear(S, nxp, nyp), linear(PT, nxp, nyp)
+    if offsets is not None:
+        soff = np.ascontiguousarray(soff[np.asarray(offsets, np.int64)])
     w = cpu_weight_table(p, nzp) if weights is None else np.ascontiguousarray(weights, np.float32)
     z0, z1 = (0, nz) if z_range is None else z_range
     shape = _i32([nx, ny, nz])
@@ -219,6 +225,25 @@ def filter_iteration(R: np.ndarray, P: np.ndarray, sigma, p: Params, *, precisio
         sg = np.ascontiguousarray(sigma, np.float64)
         orc().orc_filter_f64(Rp, Pp, shape, _i32(pad), sg, p.beta, len(soff), soff, len(poff), poff, w,
                              p.weight_mode, out, z0, z1, nthreads)
+    return out
+
+
+def weight_mass(P: np.ndarray, sigma, p: Params, tile=(32, 8, 8)) -> np.ndarray:
+    """Per-tile weight mass of every search offset ([tiles, n_search],
+    float64) for previous iterate P: the labels of the learned proposal."""
+    P = np.ascontiguousarray(P, np.float32)
+    nz, ny, nx = P.shape
+    pad = p.padding()
+    Pp = pad_reflect(P, pad)
+    nzp, nyp, nxp = Pp.shape
+    S = search_offsets(p.radius_searchspace, p.nslices)
+    PT = patch_offsets(p.radius_patchspace, p.nslices)
+    soff, poff = linear(S, nxp, nyp), linear(PT, nxp, nyp)
+    w = cpu_weight_table(p, nzp)
+    nt = -(-nx // tile[0]) * -(-ny // tile[1]) * -(-nz // tile[2])
+    out = np.zeros((nt, len(soff)), np.float64)
+    orc().orc_weight_mass_f32(Pp, _i32([nx, ny, nz]), _i32(pad), np.ascontiguousarray(sigma, np.float32), p.beta,
+                              len(soff), soff, len(poff), poff, w, p.weight_mode, _i32(list(tile)), out)
     return out
 
 
