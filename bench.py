@@ -89,10 +89,12 @@ def cpu_baseline(raw_host, rs, rp, sigma, target_s=12.0):
                       f", {cores} OpenMP threads, {dt:.1f} s"}
 
 
-def run_form(form, vol, nx, ny, nz, rs, rp, sig0, sig1, steps, warmup, log):
+def run_form(form, vol, nx, ny, nz, rs, rp, sig0, sig1, steps, warmup, log, setup=None):
     """W warmup and K timed filter iterations of one arithmetic form on the
     resident volume with the given sigma lists; returns the result tensor,
-    wall seconds of the timed steps and the filter kernel times."""
+    wall seconds of the timed steps, the filter kernel times and the table
+    sizes.  `setup(handle)` runs after the original is loaded (it attaches
+    offset pruning) and its return value is passed back as the last item."""
     import torch
 
     from microtomodenoise_amd import DenoiseParameters, IterativeNLM_GPU
@@ -104,6 +106,7 @@ def run_form(form, vol, nx, ny, nz, rs, rp, sig0, sig1, steps, warmup, log):
         raise SystemExit(f"configure_device failed ({rc})")
     n_search, n_patch = g.table_sizes()
     g.load_raw_device(vol.data_ptr())
+    extra = setup(g) if setup is not None else None
     it = 1
     for _ in range(warmup):
         g.set_sigma(sig0 if it == 1 else sig1)
@@ -123,10 +126,48 @@ def run_form(form, vol, nx, ny, nz, rs, rp, sig0, sig1, steps, warmup, log):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     log(f"[bench] {form}: {steps} timed iterations, filter {', '.join(f'{k:.1f}' for k in kms)} ms")
+    if setup is not None:
+        extra = dict(extra or {}, **g.proposal_stats())
     out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
     g.get_result_device(out.data_ptr())
     g.free_device()
-    return out, wall, kms, n_search, n_patch
+    return out, wall, kms, n_search, n_patch, extra
+
+
+def train_proposal(cfg, rs, rp, n_volumes, steps, log):
+    """The learned offset proposal for this workload: training phantoms of
+    the workload's recipe at 128x128x64 (seeds 100..), weight-mass labels of
+    the FP32 parity filter at iteration 1, Adam.  Returns (proposal, info)."""
+    import dataclasses
+
+    import torch
+
+    from microtomodenoise_amd import DenoiseParameters
+    from microtomodenoise_amd import phantom as ph
+    from microtomodenoise_amd import proposal as PR
+    params = DenoiseParameters(radius_searchspace=[rs] * 3, radius_patchspace=[rp] * 3)
+    params.cpu.blockwise = False
+    params.noiselevel.mode = "global"
+    base = ph.CONFIGS[cfg]
+    t0 = time.perf_counter()
+    data = []
+    for i in range(n_volumes):
+        spec = dataclasses.replace(base, shape=(128, 128, 64), seed=100 + i, streaks=min(base.streaks, 20),
+                                   streak_len=(10, 60))
+        vol = torch.from_numpy(ph.make(spec)).cuda()
+        mass, _ = PR.make_labels(vol, params, iteration=1)
+        data.append((vol, mass))
+    t1 = time.perf_counter()
+    net = PR.OffsetProposal(data[0][1].shape[1], seed=1)
+    hist = PR.train(net, data, steps=steps, lr=3e-3, log=log, log_every=100)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    info = {"training_volumes": f"{n_volumes} x 128x128x64 phantoms of the same recipe (seeds 100-{99 + n_volumes})",
+            "labels": "per-tile weight mass of the FP32 parity filter, iteration 1",
+            "steps": steps, "loss_first": round(hist[0], 5), "loss_last": round(float(sum(hist[-10:]) / 10), 5),
+            "label_seconds": round(t1 - t0, 2), "train_seconds": round(t2 - t1, 2),
+            "parameters": net.num_params}
+    return net, info
 
 
 def main():
@@ -135,8 +176,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
-    ap.add_argument("--form", default="bf16", choices=["bf16", "fp16", "fp32_fast"],
+    ap.add_argument("--form", default="fp16", choices=["bf16", "fp16", "fp32_fast"],
                     help="accelerated form measured against the plain FP32 path")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned offset proposal runs")
+    ap.add_argument("--proposal-k", type=int, default=128)
+    ap.add_argument("--proposal-volumes", type=int, default=4)
+    ap.add_argument("--proposal-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -191,22 +236,28 @@ def main():
     sig1 = (np.float32(params.alpha) * s0 + (np.float32(1) - np.float32(params.alpha)) * s1).astype(np.float32)
     log(f"[bench] sigma0 {s0[0]:.3f}, sigma1 {sig1[0]:.3f}")
 
-    out_p, wall_p, kms_p, n_search, n_patch = run_form("fp32", vol, nx, ny, nz, rs, rp, s0, sig1, args.steps,
-                                                       args.warmup, log)
-    out_f, wall_f, kms_f, _, _ = run_form(args.form, vol, nx, ny, nz, rs, rp, s0, sig1, args.steps, args.warmup,
-                                          log)
+    out_p, wall_p, kms_p, n_search, n_patch, _ = run_form("fp32", vol, nx, ny, nz, rs, rp, s0, sig1, args.steps,
+                                                          args.warmup, log)
+    out_f, wall_f, kms_f, _, _, _ = run_form(args.form, vol, nx, ny, nz, rs, rp, s0, sig1, args.steps, args.warmup,
+                                             log)
     V = nx * ny * nz
     clean_t = torch.from_numpy(clean).to("cuda")
     peak = 65535.0
-    mse_p = float(torch.mean((out_p.double() - clean_t.double()) ** 2))
-    mse_f = float(torch.mean((out_f.double() - clean_t.double()) ** 2))
-    mse_n = float(torch.mean((vol.double() - clean_t.double()) ** 2))
     psnr = lambda m: 10.0 * np.log10(peak * peak / m)  # noqa: E731
+
+    def mse_clean(out):
+        return float(torch.mean((out.double() - clean_t.double()) ** 2))
+
+    def vs_plain(out):
+        diff = (out.double() - out_p.double())
+        r = float(torch.sqrt(torch.mean(diff ** 2))), float(torch.max(torch.abs(diff)))
+        del diff
+        return r
+
+    mse_p, mse_f, mse_n = mse_clean(out_p), mse_clean(out_f), mse_clean(vol)
     d_psnr = psnr(mse_p) - psnr(mse_f)
-    diff = (out_f.double() - out_p.double())
-    rmse_fp = float(torch.sqrt(torch.mean(diff ** 2)))
-    maxabs_fp = float(torch.max(torch.abs(diff)))
-    del diff, clean_t
+    rmse_fp, maxabs_fp = vs_plain(out_f)
+    del out_f
     log(f"[bench] PSNR noisy {psnr(mse_n):.3f} dB, plain {psnr(mse_p):.3f} dB, {args.form} {psnr(mse_f):.3f} dB "
         f"(loss {d_psnr:.4f} dB); {args.form} vs plain RMSE {rmse_fp:.3f} max {maxabs_fp:.3f}")
 
@@ -214,6 +265,49 @@ def main():
     v_fast = V * args.steps / wall_f / 1e6
     within = d_psnr <= 0.05
     F = n_search * (2 * n_patch + 8)
+
+    # learned offset proposal: trained on small phantoms of this recipe, then
+    # the pruned matrix-core path with the 0.95 predicted-mass fallback and a
+    # 2 % audit, and (for the trade-off) the same lists with no fallback
+    pruned = None
+    if not args.no_proposal:
+        net, pinfo = train_proposal(cfg, rs, rp, args.proposal_volumes, args.proposal_steps, log)
+
+        def attach(threshold):
+            def setup(g):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                g.set_proposal(net, args.proposal_k, threshold=threshold, audit_frac=0.02, seed=7)
+                return {"cnn_inference_ms": round((time.perf_counter() - t) * 1e3, 2)}
+            return setup
+
+        pruned = {"K": args.proposal_k, "training": pinfo,
+                  "cnn_flops_per_voxel": 2 * 27 * 16 + 3 * 2 * 27 * 16 * 16}
+        for name, thr in (("fallback_0.95", 0.95), ("no_fallback", 0.0)):
+            out_l, wall_l, kms_l, _, _, st = run_form(args.form, vol, nx, ny, nz, rs, rp, s0, sig1, args.steps,
+                                                      args.warmup, log, setup=attach(thr))
+            mse_l = mse_clean(out_l)
+            rmse_l, maxabs_l = vs_plain(out_l)
+            del out_l
+            v_l = V * args.steps / wall_l / 1e6
+            k_eff = st["mean_offsets_visited"]
+            pruned[name] = {
+                "threshold": thr, "audit_frac": 0.02, "value": round(v_l, 3),
+                "ms_per_step": round(wall_l / args.steps * 1e3, 3),
+                "speedup_vs_plain": round(v_l / v_plain, 3),
+                "psnr_db": round(psnr(mse_l), 4), "psnr_loss_db": round(psnr(mse_p) - psnr(mse_l), 4),
+                "rmse_vs_plain": round(rmse_l, 4), "maxabs_vs_plain": round(maxabs_l, 4),
+                "fallback_tiles_frac": round(st["fallback_tiles"] / max(st["tiles"], 1), 4),
+                "audited_tiles_frac": round(st["audited_tiles"] / max(st["tiles"], 1), 4),
+                "mean_predicted_mass_kept": round(st["mean_predicted_mass"], 4),
+                "k_eff": round(k_eff, 2), "kernel_ms": round(float(np.mean(kms_l)), 3),
+                "cnn_inference_ms": st["cnn_inference_ms"],
+                "flops_per_voxel_iter": round(k_eff * (2 * n_patch + 8), 1)}
+            log(f"[bench] learned pruning ({name}): {v_l:.1f} Mvoxel-iter/s, PSNR loss "
+                f"{pruned[name]['psnr_loss_db']:.4f} dB, fallback {pruned[name]['fallback_tiles_frac']:.3f}, "
+                f"K_eff {k_eff:.1f}, CNN {st['cnn_inference_ms']:.1f} ms")
+        net.close()
+    del clean_t
 
     def roof(kernel, kms):
         km = float(np.mean(kms))
@@ -237,34 +331,47 @@ def main():
         log("[bench] CPU baseline")
         cpu = cpu_baseline(raw_host, rs, rp, float(sig1[nz // 2]), args.cpu_seconds)
 
-    value = v_fast if within else v_plain
-    wall = wall_f if within else wall_p
+    # the reported value: the fastest path within the 0.05 dB budget
+    cands = [("fp32 parity", v_plain, wall_p, plain_kernel, kms_p, "f32")]
+    if within:
+        cands.append((args.form, v_fast, wall_f, fast_kernel, kms_f, {"bf16": "bf16", "fp16": "f16"}.get(args.form,
+                                                                                                       "f32")))
+    if pruned:
+        for name in ("fallback_0.95", "no_fallback"):
+            r = pruned[name]
+            if r["psnr_loss_db"] <= 0.05:
+                cands.append((f"{args.form} + learned pruning ({name})", r["value"], args.steps / r["value"] * V / 1e6,
+                              fast_kernel, [r["kernel_ms"]], {"bf16": "bf16", "fp16": "f16"}.get(args.form, "f32")))
+    best = max(cands, key=lambda c: c[1])
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(best[1], 3),
         "unit": "Mvoxel-iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(best[2] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if (within and args.form == "bf16") else ("f16" if within and args.form == "fp16" else "f32"),
+        "dtype": best[5],
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}", "shape_xyz": [nx, ny, nz], "patch_radius": rp,
                    "search_radius": rs, "n_offsets": n_search, "n_taps": n_patch,
-                   "form": args.form if within else "fp32 parity", "iterations": args.warmup + args.steps,
-                   "noise": "global, sigma lists from the GPU estimator, shared by both forms",
-                   "plain_fp32": {"value": round(v_plain, 3), "ms_per_step": round(wall_p / args.steps * 1e3, 3)},
+                   "form": best[0], "iterations": args.warmup + args.steps,
+                   "noise": "global, sigma lists from the GPU estimator, shared by every form",
+                   "psnr_noisy_db": round(psnr(mse_n), 4),
+                   "plain_fp32": {"value": round(v_plain, 3), "ms_per_step": round(wall_p / args.steps * 1e3, 3),
+                                  "psnr_db": round(psnr(mse_p), 4)},
                    "accelerated": {"form": args.form, "value": round(v_fast, 3),
                                    "ms_per_step": round(wall_f / args.steps * 1e3, 3),
                                    "speedup_vs_plain": round(v_fast / v_plain, 3),
                                    "psnr_plain_db": round(psnr(mse_p), 4), "psnr_db": round(psnr(mse_f), 4),
                                    "psnr_loss_db": round(d_psnr, 4), "rmse_vs_plain": round(rmse_fp, 4),
                                    "maxabs_vs_plain": round(maxabs_fp, 4), "within_budget": bool(within),
-                                   "offset_pruning": "none (full search)"}},
-        "roofline": roof(fast_kernel if within else plain_kernel, kms_f if within else kms_p),
+                                   "offset_pruning": "none (full search)"},
+                   "learned_pruning": pruned},
+        "roofline": roof(best[3], best[4]),
         "roofline_plain_fp32": roof(plain_kernel, kms_p),
         "cpu_baseline": cpu,
     }

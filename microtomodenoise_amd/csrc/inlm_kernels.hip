@@ -239,9 +239,15 @@ filter_tiled_fp32(TiledArgs a)
     const float m = -1.f / (sig * sig * a.beta);
     const bool valid = y < g.ny && z < g.nz;
 
-    float fv[VX], fw[VX], wmax[VX];
+    // wmax as the bits of a non-negative float: the integer max orders those
+    // like floats (== (wt > wmax ? wt : wmax)) without NaN canonicalisation
+    float fv[VX], fw[VX];
+    unsigned wmax_bits[VX];
 #pragma unroll
-    for (int v = 0; v < VX; v++) fv[v] = fw[v] = wmax[v] = 0.f;
+    for (int v = 0; v < VX; v++) {
+        fv[v] = fw[v] = 0.f;
+        wmax_bits[v] = 0u;
+    }
 
     const int ldsrow = ((tz + R) * T::RY + (ty + R)) * T::PITCH + VX * xr;
     const int nS = a.n_search;
@@ -336,7 +342,7 @@ filter_tiled_fp32(TiledArgs a)
             const float wt = weight_of<WFN>(d[v] * m);
             fv[v] = fv[v] + wt * rs[v];
             fw[v] = fw[v] + wt;
-            wmax[v] = __builtin_fmaxf(wmax[v], wt);  // == (wt > wmax ? wt : wmax): weights are >= 0, never NaN
+            wmax_bits[v] = __builtin_elementwise_max(wmax_bits[v], __builtin_bit_cast(unsigned, wt));
             if constexpr (MASS) sw += (valid && x0 + v < g.nx) ? wt : 0.f;
         }
         if constexpr (MASS) {
@@ -358,6 +364,9 @@ filter_tiled_fp32(TiledArgs a)
         }
     }
 
+    float wmax[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) wmax[v] = __builtin_bit_cast(float, wmax_bits[v]);
     if (y < g.ny && z < g.nz) {
         float *o = a.out + ((long)z * g.ny + y) * g.nx + x0;
 #pragma unroll

@@ -38,15 +38,18 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Patch rows of the isotropic ellipsoid of radius R and their weight
 // patterns: a row's weights as a function of dx depend only on (dy^2, dz^2),
-// so rows sharing that key share one B operand.
+// so rows sharing that key share one B operand.  When the weights depend on
+// dx^2+dy^2+dz^2 alone (no z anisotropy), patterns of equal dy^2+dz^2 carry
+// the same band too and share a B operand class.
 struct MRow {
     int dz, dy, j, pat;
 };
 template <int R>
 struct MRows {
     MRow r[4 * R * R + 4 * R + 1];
-    int n, npat;
+    int n, npat, ncls;
     int pdy[kMfmaMaxPatterns], pdz[kMfmaMaxPatterns], pj[kMfmaMaxPatterns];
+    int cls[kMfmaMaxPatterns], crep[kMfmaMaxPatterns], ckey[kMfmaMaxPatterns];
     static constexpr bool inside(int x, int y, int z)
     {
         const float rr = (float)R;
@@ -56,7 +59,7 @@ struct MRows {
         e = e + fz * fz;
         return e <= 1.f;
     }
-    constexpr MRows() : r(), n(0), npat(0), pdy(), pdz(), pj()
+    constexpr MRows() : r(), n(0), npat(0), ncls(0), pdy(), pdz(), pj(), cls(), crep(), ckey()
     {
         for (int z = -R; z <= R; ++z)
             for (int y = -R; y <= R; ++y) {
@@ -75,6 +78,18 @@ struct MRows {
                 }
                 r[n++] = MRow{z, y, j, p};
             }
+        for (int p = 0; p < npat; p++) {
+            const int key = pdy[p] * pdy[p] + pdz[p] * pdz[p];
+            int c = -1;
+            for (int q = 0; q < ncls; q++)
+                if (ckey[q] == key) c = q;
+            if (c < 0) {
+                c = ncls++;
+                ckey[c] = key;
+                crep[c] = p;
+            }
+            cls[p] = c;
+        }
     }
 };
 template <int R>
@@ -174,9 +189,14 @@ __device__ __forceinline__ float fast_weight(float a)
 // can never leave the allocation the resource describes.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer, 32-bit data format
+// `bytes` is the distance from base to the end of the allocation; callers
+// only form bases inside their allocation (a tile origin plus an offset of
+// the padded volume, or the guard in front of it), so it is positive, and
+// the unsigned test keeps the clamp on the scalar unit.
 __device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes)
 {
-    const int n = bytes <= 0 ? 0 : (bytes > 0x7fffffffL ? 0x7fffffff : (int)bytes);
+    const unsigned long b = (unsigned long)bytes;
+    const int n = (b >> 31) != 0 ? 0x7fffffff : (int)b;
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
 }
 template <class TS>
@@ -274,7 +294,7 @@ struct Stager<R, _Float16, _Float16> {
 template <int R>
 constexpr int mfma_waves_per_eu() { return R <= 2 ? 6 : 4; }
 
-template <int R, class TS, class TD, int WFN, bool PRUNED>
+template <int R, class TS, class TD, int WFN, bool PRUNED, bool ISO>
 __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R>(), mfma_waves_per_eu<R>())))
 void filter_mfma(MfmaArgs<TS> a)
 {
@@ -299,17 +319,20 @@ void filter_mfma(MfmaArgs<TS> a)
     // zeros (they meet zero weights in the band, and 0 * NaN would poison).
     for (int i = t; i < T::LDS; i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
 
-    // B operands, one band per pattern.  Lane l holds B[k][n] for n = l % 16
-    // and k = 8*(l/16) + i, i.e. input column x0 - 8 + k, so dx = k - 8 - n.
+    // B operands, one band per pattern (per pattern class when ISO).  Lane l
+    // holds B[k][n] for n = l % 16 and k = 8*(l/16) + i, i.e. input column
+    // x0 - 8 + k, so dx = k - 8 - n.
     const int n = lane & 15, q = lane >> 4;
-    vec8 bop[rows.npat];
+    constexpr int NB = ISO ? rows.ncls : rows.npat;
+    vec8 bop[NB];
 #pragma unroll
-    for (int p = 0; p < rows.npat; p++) {
+    for (int b = 0; b < NB; b++) {
+        const int p = ISO ? rows.crep[b] : b;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int dx = 8 * q + i - 8 - n;
             const float w = (dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f;
-            bop[p][i] = DP::from(w);
+            bop[b][i] = DP::from(w);
         }
     }
 
@@ -349,7 +372,10 @@ void filter_mfma(MfmaArgs<TS> a)
     const int am = lane & 15;
     const int aoff = ((2 * gz + (am >> 3) + R) * T::RY + ((am & 7) + R)) * XW + 16 * xh + 8 * q;
 
-    float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f}, wmax[4] = {0.f, 0.f, 0.f, 0.f};
+    // wmax is kept as the bit pattern of a non-negative float: integer max
+    // orders those like floats and needs no NaN canonicalisation
+    float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
+    unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
 
     typedef typename ST::chunk chunk;
     auto load_chunks = [&](chunk (&pn)[T::CPT], int lin) __attribute__((always_inline)) {
@@ -407,22 +433,42 @@ void filter_mfma(MfmaArgs<TS> a)
 
         const TD *L = lds[k & 1] + aoff;
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-        mstatic_for<0, rows.n>([&](auto ri) __attribute__((always_inline)) {
+        // A operands in flight: KD reads ahead of the MFMA that consumes them,
+        // so the chain waits on LDS latency once rather than per row
+        constexpr int NR = rows.n;
+        constexpr int KD = NR < 3 ? NR : 3;
+        auto a_of = [&](auto ri) __attribute__((always_inline)) {
             constexpr MRow rw = MRowsOf<R>::value.r[decltype(ri)::value];
-            const vec8 av = *reinterpret_cast<const vec8 *>(L + (rw.dz * T::RY + rw.dy) * XW);
-            acc = DP::mfma(av, bop[rw.pat], acc);
+            return *reinterpret_cast<const vec8 *>(L + (rw.dz * T::RY + rw.dy) * XW);
+        };
+        vec8 ab[KD];
+        mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = a_of(ri); });
+        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+            constexpr int r = decltype(ri)::value;
+            constexpr MRow rw = MRowsOf<R>::value.r[r];
+            acc = DP::mfma(ab[r % KD], bop[ISO ? MRowsOf<R>::value.cls[rw.pat] : rw.pat], acc);
+            if constexpr (r + KD < NR) ab[r % KD] = a_of(std::integral_constant<int, r + KD>{});
+        });
+        // keep that order: KD reads, then (MFMA, read) pairs, then the tail
+        __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
+        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+            __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
+            if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
         });
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const float w = fast_weight<WFN>(acc[i] * mm);
             fv[i] = __builtin_fmaf(w, rv[i], fv[i]);
             fw[i] = fw[i] + w;
-            wmax[i] = __builtin_fmaxf(wmax[i], w);
+            wmax_bits[i] = __builtin_elementwise_max(wmax_bits[i], __builtin_bit_cast(unsigned, w));
         }
         if (more) stage(pn, lds[(k + 1) & 1]);
         __syncthreads();
     }
 
+    float wmax[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) wmax[i] = __builtin_bit_cast(float, wmax_bits[i]);
     if (x < g.nx && z < g.nz) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -549,10 +595,17 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const flo
     if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
     const long nwg = (long)a.ntx * a.nty * a.ntz;
     if (ol && (ol->ntiles != nwg || !ol->lists || !ol->counts || ol->K < 1)) return hipErrorInvalidValue;
-    if (ol)
-        hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
-    else
-        hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, false>), dim3((unsigned)nwg), dim3(NTM), 0, st, a);
+    // patterns of one class carry identical bands when the weights are
+    // isotropic: then one B operand serves the class
+    constexpr const MRows<R> &rows = MRowsOf<R>::value;
+    bool iso = true;
+    for (int p = 0; p < rows.npat; p++)
+        for (int i = 0; i < 8; i++) iso = iso && a.band[p][i] == a.band[rows.crep[rows.cls[p]]][i];
+    const dim3 grid((unsigned)nwg), block(NTM);
+    if (ol && iso) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, true>), grid, block, 0, st, a);
+    else if (ol) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, false>), grid, block, 0, st, a);
+    else if (iso) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, false, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, false, false>), grid, block, 0, st, a);
     return hipGetLastError();
 }
 

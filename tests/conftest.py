@@ -26,6 +26,12 @@ def filter_fixtures():
 
 @pytest.fixture(scope="session")
 def gpu_available():
+    # torch (which some GPU tests use for device buffers) carries its own HIP
+    # runtime; it has to initialise the device before the library's runtime
+    # does, as in bench.py
+    import torch
+    torch.cuda.init()
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU: torch reports none"
     from microtomodenoise_amd import _capi
     n = _capi.lib().inlm_device_count()
     assert n >= 1, "GPU tests need a visible GPU: the HIP library reports none"
