@@ -38,7 +38,8 @@ typedef enum {
     INLM_NO_DEVICE = -1,
     INLM_HIP_ERROR = -2,
     INLM_BAD_ARG = -3,
-    INLM_OOM = -4
+    INLM_OOM = -4,
+    INLM_IO_ERROR = -5 /* a file could not be read, parsed or written */
 } inlm_status;
 
 /* NoiseLevel.mode of denoise_parameters.h:23 ("global"/"auto",
@@ -186,6 +187,12 @@ INLM_API float inlm_last_filter_ms(const inlm_handle *h);
 /* Table sizes of a handle: offsets visited and patch taps. */
 INLM_API inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n_patch);
 
+/* Makes `vol` (nx*ny*nz floats, host or device) the result of iteration
+ * `iter` without filtering: the resume path of Run_GaussianNoise
+ * (iternlm_gpu.cu:1665-1697), which loads a finished iteration from disk
+ * into the device result.  The original must be loaded. */
+INLM_API inlm_status inlm_set_result(inlm_handle *h, int32_t iter, const float *vol, int32_t is_device);
+
 /* ---------------------------------------------------------------------------
  * Learned offset proposal.  None of these has a counterpart in the reference:
  * they serve the pruned matrix-core path, in which each 32x8x8 filter tile
@@ -248,6 +255,58 @@ INLM_API inlm_status inlm_set_offset_lists(inlm_handle *h, const int32_t *lists,
  * tiles, mean predicted mass of the kept offsets, mean offsets visited per
  * tile. */
 INLM_API inlm_status inlm_proposal_stats(const inlm_handle *h, float out[5]);
+
+/* ---------------------------------------------------------------------------
+ * Volume I/O and the command-line program's output contract
+ * (Geometry/hdcommunication.cpp, main.cpp:231-563).  Host-only except
+ * inlm_denoise_files; baseline uncompressed TIFF without libtiff.
+ * ------------------------------------------------------------------------- */
+
+/* Shape {nx, ny, nz} of a TIFF input: a directory of slice files (*.tif,
+ * *.tiff, sorted by name) or one file with one image per IFD or an ImageJ
+ * stack (GetFilelist_And_ImageSequenceDimensions, hdcommunication.cpp:51-104). */
+INLM_API inlm_status inlm_tiff_stack_shape(const char *path, int32_t shape[3]);
+
+/* Slices first..last of a TIFF input as float32 (8/16-bit unsigned samples
+ * converted, 32-bit samples taken as IEEE floats: Get3DTifSequence_32bitPointer,
+ * hdcommunication.cpp:975-1060).  out holds nx*ny*(last-first+1) floats. */
+INLM_API inlm_status inlm_tiff_read_stack(const char *path, int32_t first, int32_t last, float *out);
+
+/* One TIFF per slice, dir/name<z0+i, 4 digits>.tif, float32 or (as16bit)
+ * uint16 = min(max(0, round(x)), 65535) (SaveTifSequence_32bit / _as16bit,
+ * hdcommunication.cpp:1550-1627); creates dir. */
+INLM_API inlm_status inlm_tiff_write_stack(const char *dir, const char *name, const float *vol,
+                                           const int32_t shape[3], int32_t z0, int32_t as16bit);
+
+/* Noise files "i,sigma" per line (SaveNoisefile / ReadNoisefile,
+ * hdcommunication.cpp:1736-1786); reading fails unless there are nz lines. */
+INLM_API inlm_status inlm_noise_csv_write(const char *dir, const char *name, const float *sigma, int32_t n);
+INLM_API inlm_status inlm_noise_csv_read(const char *file, int32_t nz, float *sigma);
+
+/* Resume check of iternlm_cpu.cpp:51-85: 1 when dir holds denoised<first..last>.tif
+ * and no file is smaller than one before it. */
+INLM_API int32_t inlm_iteration_on_disk(const char *dir, int32_t first, int32_t last);
+
+/* Options of the command-line program (protocol::DenoiseParameters::io,
+ * denoise_parameters.h). firstslice/lastslice -1: the whole stack. */
+typedef struct inlm_io_options {
+    int32_t save_16bit; /* --16bit: uint16 output */
+    int32_t resume;     /* --resume: reuse finished iterations and noise files */
+    int32_t cleanup;    /* keep only the last iteration, in outpath itself */
+    int32_t firstslice, lastslice;
+    int32_t verbose;
+    int32_t reserved[8];
+} inlm_io_options;
+INLM_API void inlm_default_io_options(inlm_io_options *o);
+
+/* The whole program of main.cpp:231-563 on one GPU: read the TIFF input,
+ * estimate sigma0 (noise_vs_slice00.csv), iteration 1 into
+ * outpath/denoise-iteration01/denoised<z>.tif, sigma1 (noise_vs_slice01.csv)
+ * blended with alpha, iterations 2..maxiterations, resume and cleanup as the
+ * reference, and a line block appended to logfile.txt next to outpath.
+ * outpath NULL or empty: <parent of inpath>/denoised/. */
+INLM_API inlm_status inlm_denoise_files(const inlm_params *p, int32_t device, const char *inpath,
+                                        const char *outpath, const inlm_io_options *o);
 
 #ifdef __cplusplus
 }

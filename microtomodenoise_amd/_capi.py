@@ -14,12 +14,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libinlm.so")
 
-INLM_OK, INLM_NO_DEVICE, INLM_HIP_ERROR, INLM_BAD_ARG, INLM_OOM = 0, -1, -2, -3, -4
+INLM_OK, INLM_NO_DEVICE, INLM_HIP_ERROR, INLM_BAD_ARG, INLM_OOM, INLM_IO_ERROR = 0, -1, -2, -3, -4, -5
 NOISE_GLOBAL, NOISE_ZADAPTIVE, NOISE_MANUAL = 0, 1, 2
 PREC_FP32, PREC_FP32_FAST, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
 WEIGHT_PADE, WEIGHT_EXP = 0, 1
 
-STATUS_NAMES = {0: "OK", -1: "NO_DEVICE", -2: "HIP_ERROR", -3: "BAD_ARG", -4: "OOM"}
+STATUS_NAMES = {0: "OK", -1: "NO_DEVICE", -2: "HIP_ERROR", -3: "BAD_ARG", -4: "OOM", -5: "IO_ERROR"}
 
 # Every symbol include/inlm_capi.h declares.
 EXPORTS = [
@@ -32,7 +32,24 @@ EXPORTS = [
     "inlm_proposal_last_error", "inlm_proposal_num_params", "inlm_proposal_params", "inlm_proposal_grads",
     "inlm_proposal_train_step", "inlm_proposal_predict", "inlm_set_proposal", "inlm_set_offset_lists",
     "inlm_proposal_stats",
+    # resume, volume I/O and the program's output contract
+    "inlm_set_result", "inlm_tiff_stack_shape", "inlm_tiff_read_stack", "inlm_tiff_write_stack",
+    "inlm_noise_csv_write", "inlm_noise_csv_read", "inlm_iteration_on_disk", "inlm_default_io_options",
+    "inlm_denoise_files",
 ]
+
+
+class InlmIoOptions(C.Structure):
+    """Mirror of ``inlm_io_options`` (include/inlm_capi.h)."""
+    _fields_ = [
+        ("save_16bit", C.c_int32),
+        ("resume", C.c_int32),
+        ("cleanup", C.c_int32),
+        ("firstslice", C.c_int32),
+        ("lastslice", C.c_int32),
+        ("verbose", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
 
 
 class InlmParams(C.Structure):
@@ -116,6 +133,18 @@ def lib():
         L.inlm_set_proposal.argtypes = [H, H, C.c_int32, C.c_float, C.c_float, C.c_uint64]
         L.inlm_set_offset_lists.argtypes = [H, i32p, i32p, C.c_int32]
         L.inlm_proposal_stats.argtypes = [H, f32p]
+        L.inlm_set_result.argtypes = [H, C.c_int32, C.c_void_p, C.c_int32]
+        L.inlm_tiff_stack_shape.argtypes = [C.c_char_p, i32p]
+        L.inlm_tiff_read_stack.argtypes = [C.c_char_p, C.c_int32, C.c_int32, f32p]
+        L.inlm_tiff_write_stack.argtypes = [C.c_char_p, C.c_char_p, f32p, i32p, C.c_int32, C.c_int32]
+        L.inlm_noise_csv_write.argtypes = [C.c_char_p, C.c_char_p, f32p, C.c_int32]
+        L.inlm_noise_csv_read.argtypes = [C.c_char_p, C.c_int32, f32p]
+        L.inlm_iteration_on_disk.argtypes = [C.c_char_p, C.c_int32, C.c_int32]
+        L.inlm_iteration_on_disk.restype = C.c_int32
+        L.inlm_default_io_options.argtypes = [C.POINTER(InlmIoOptions)]
+        L.inlm_default_io_options.restype = None
+        L.inlm_denoise_files.argtypes = [C.POINTER(InlmParams), C.c_int32, C.c_char_p, C.c_char_p,
+                                         C.POINTER(InlmIoOptions)]
         _LIB = L
     return _LIB
 

@@ -11,16 +11,21 @@
 #include <string>
 #include <vector>
 
+#include "inlm_internal.hpp"
 #include "inlm_kernels.hpp"
 #include "inlm_learn.hpp"
 #include "inlm_tables.hpp"
 
 namespace {
 
-std::string g_create_error;  // message of the last failed inlm_create
+std::string g_create_error;  // message of the last failed call without a handle
 constexpr size_t kGuard = 64;
 
 }  // namespace
+
+namespace inlm {
+void set_global_error(const std::string &msg) { g_create_error = msg; }
+}  // namespace inlm
 
 struct inlm_handle {
     inlm_params p{};
@@ -671,6 +676,19 @@ inlm_status inlm_table_sizes(const inlm_handle *h, int32_t *n_search, int32_t *n
     if (!h) return INLM_BAD_ARG;
     if (n_search) *n_search = (int32_t)h->search.size();
     if (n_patch) *n_patch = h->taps.n;
+    return INLM_OK;
+}
+
+inlm_status inlm_set_result(inlm_handle *h, int32_t iter, const float *vol, int32_t is_device)
+{
+    if (!h || !vol) return fail(h, INLM_BAD_ARG, "null argument");
+    if (iter < 1) return fail(h, INLM_BAD_ARG, "iterations count from 1");
+    if (!h->raw_loaded) return fail(h, INLM_BAD_ARG, "no original volume loaded");
+    HIP_TRY(h, hipSetDevice(h->device));
+    HIP_TRY(h, hipMemcpyAsync(h->result, vol, sizeof(float) * h->geo.voxels(),
+                              is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->last_iter = iter;
     return INLM_OK;
 }
 

@@ -29,7 +29,15 @@ namespace {
 
 constexpr int NTM = 512;
 constexpr int MTX = 32, MTY = 8, MTZ = 8;
-constexpr int XW = 56;  // LDS row pitch (elements); columns [0,4) and [44,56) stay zero
+// D-plane layout in LDS: rows of XW = 48 16-bit elements (columns [0,4) and
+// [44,48) stay zero), 16 rows per slice whatever the patch radius.  With a
+// 96-byte row pitch and a 1536-byte slice pitch the 16-lane groups in which
+// gfx950 serves a ds_read_b128 (64 banks, 256 bytes per clock) hit 16
+// distinct 16-byte bank groups for the A-operand pattern (2 slices x 8 rows
+// x 4 column blocks), so the reads run conflict-free; the former 112-byte
+// pitch with 12 rows per slice took three times as many LDS cycles per read.
+constexpr int XW = 48;
+constexpr int RYP = 16;
 constexpr float kCut = -3.56648f;
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -160,7 +168,8 @@ struct MTile {
     static constexpr int CPR = 10;  // 4-column chunks staged per row: columns [4, 44)
     static constexpr int NCHUNK = RZ * RY * CPR;
     static constexpr int CPT = (NCHUNK + NTM - 1) / NTM;
-    static constexpr int LDS = RZ * RY * XW;  // elements per buffer
+    static constexpr int LDS = RZ * RYP * XW;  // elements per buffer
+    static_assert(RY <= RYP, "patch radius too large for the padded slice");
 };
 
 __device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
@@ -349,7 +358,7 @@ void filter_mfma(MfmaArgs<TS> a)
         const int cx = ci % T::CPR, row = ci / T::CPR;
         const int ry = row % T::RY, rz = row / T::RY;
         coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
-        loff[c] = row * XW + 4 + 4 * cx;
+        loff[c] = (rz * RYP + ry) * XW + 4 + 4 * cx;
         stg.load(stg.p0[c], Pt + coff[c]);
     }
 
@@ -370,7 +379,7 @@ void filter_mfma(MfmaArgs<TS> a)
     // A operand: lane l reads row m = l % 16 (slice pair member m/8, row m%8),
     // columns 16*xh + 8*q .. +7.
     const int am = lane & 15;
-    const int aoff = ((2 * gz + (am >> 3) + R) * T::RY + ((am & 7) + R)) * XW + 16 * xh + 8 * q;
+    const int aoff = ((2 * gz + (am >> 3) + R) * RYP + ((am & 7) + R)) * XW + 16 * xh + 8 * q;
 
     // wmax is kept as the bit pattern of a non-negative float: integer max
     // orders those like floats and needs no NaN canonicalisation
@@ -439,7 +448,7 @@ void filter_mfma(MfmaArgs<TS> a)
         constexpr int KD = NR < 3 ? NR : 3;
         auto a_of = [&](auto ri) __attribute__((always_inline)) {
             constexpr MRow rw = MRowsOf<R>::value.r[decltype(ri)::value];
-            return *reinterpret_cast<const vec8 *>(L + (rw.dz * T::RY + rw.dy) * XW);
+            return *reinterpret_cast<const vec8 *>(L + (rw.dz * RYP + rw.dy) * XW);
         };
         vec8 ab[KD];
         mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = a_of(ri); });
@@ -455,6 +464,12 @@ void filter_mfma(MfmaArgs<TS> a)
             __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
             if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
         });
+        // The vector ALU reads the matrix-core result without an interlock;
+        // the compiler's wait-state count missed the branch path that jumps
+        // from the last MFMA straight to the weights (last offset, no
+        // staging), so 16 wait states are spelled out here, ahead of any
+        // reader of acc (more than the 8-pass MFMA needs).
+        asm volatile("s_nop 7\n\ts_nop 7" : "+v"(acc));
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const float w = fast_weight<WFN>(acc[i] * mm);

@@ -182,6 +182,12 @@ class IterativeNLM_GPU:
     def get_result_device(self, dev_ptr: int):
         self._check(capi.lib().inlm_get_result(self._h, C.c_void_p(dev_ptr), 1))
 
+    def set_result(self, iter: int, vol):
+        """Make `vol` the result of iteration `iter` (the resume path of
+        Run_GaussianNoise, iternlm_gpu.cu:1665-1697)."""
+        v = _f32(vol)
+        self._check(capi.lib().inlm_set_result(self._h, iter, v.ctypes.data_as(C.c_void_p), 0))
+
     def load_raw(self, instack):
         raw = _f32(instack)
         self._check(capi.lib().inlm_load_raw(self._h, raw.ctypes.data_as(C.c_void_p), 0))

@@ -13,6 +13,12 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+    # torch carries its own HIP runtime, which has to reach the device before
+    # the library's does (bench.py keeps the same order); on a host without a
+    # GPU this only finds none
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.init()
 
 
 def load_npz(name):

@@ -168,8 +168,11 @@ def test_pruned_lists_match_oracle_per_tile(gpu_available, form):
     print(f"[gpu]   max {mx:.3e} rms {rms:.3e}", flush=True)
     bmax, brms = PRUNED_BUDGET[form]
     assert mx <= bmax and rms <= brms
-    # the lists matter: a tile filtered with another list is far off
-    assert np.abs(refs[0] - refs[1]).max() / spec.noise_sigma > 2 * bmax
+    # the lists matter: filtering with the other list moves the result by far
+    # more than the form's rounding does
+    sep = float(np.sqrt(np.mean(((refs[0] - refs[1]) / spec.noise_sigma) ** 2)))
+    print(f"[gpu]   rms difference between the two lists' results {sep:.3e}", flush=True)
+    assert sep > 10 * rms
 
 
 def test_proposal_fallback_audit_and_topk(gpu_available):
