@@ -50,6 +50,7 @@ struct inlm_handle {
     bool fast = false;
     double shift = 0.0, scale = 1.0;
     inlm::SearchEntry *search_dev = nullptr;
+    int min_lin = 0;  // smallest linear search offset
     int64_t *samples_dev = nullptr;
     double *std_dev = nullptr;
     int64_t samples_cap = 0;
@@ -264,6 +265,7 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         return fail(nullptr, INLM_BAD_ARG, "device index out of range");
     }
     for (const auto &o : S) h->search.push_back({o.dx, o.dy, o.dz, (int)(o.dz * sp + (long)o.dy * g.nxp + o.dx)});
+    for (const auto &e : h->search) h->min_lin = std::min(h->min_lin, e.lin);
     h->taps.n = (int)PT.size();
     for (size_t i = 0; i < PT.size(); i++) {
         h->taps.w[i] = W[i];
@@ -450,7 +452,8 @@ inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, f
         HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems,
                                             h->raw, h->result, h->sigma, h->p.beta,
                                             (float)(1.0 / (h->scale * h->scale)), h->shift, 1.0 / h->scale,
-                                            h->search_dev, ns, h->taps, h->fg, h->stream, pruned ? &ol : nullptr));
+                                            h->search_dev, ns, h->min_lin, h->taps, h->fg, h->stream,
+                                            pruned ? &ol : nullptr));
     } else {
         HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad,
                                        static_cast<const float *>(P), h->result, h->sigma, h->p.beta, h->search_dev,

@@ -69,6 +69,7 @@ hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice
 bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n);
 // elems: allocated elements of each padded volume from Rs/Ps on (bounds the
 // buffer loads).
+// min_lin: the smallest linear offset of the search table (<= 0).
 // Per-tile offset lists for the matrix-core filter (32x8x8 tiles, x fastest):
 // tile t visits the search entries lists[t*K .. t*K + counts[t]) when
 // counts[t] < n_search, else the whole search.  Device pointers.
@@ -81,7 +82,7 @@ struct OffsetLists {
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
                               const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
-                              const TapTable &taps, const FilterGeom &g, hipStream_t st,
+                              int min_lin, const TapTable &taps, const FilterGeom &g, hipStream_t st,
                               const OffsetLists *ol = nullptr);
 hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
                                 hipStream_t st);

@@ -160,6 +160,7 @@ struct MfmaArgs {
     const int *lists;
     const int *counts;
     int K;
+    int min_lin;  // smallest linear search offset (<= 0): the tile's buffer ranges start there
 };
 
 template <int R>
@@ -168,7 +169,10 @@ struct MTile {
     static constexpr int CPR = 10;  // 4-column chunks staged per row: columns [4, 44)
     static constexpr int NCHUNK = RZ * RY * CPR;
     static constexpr int CPT = (NCHUNK + NTM - 1) / NTM;
-    static constexpr int LDS = RZ * RYP * XW;  // elements per buffer
+    // staging slots past NCHUNK write to their own 4 elements behind the
+    // plane, so every thread stages CPT chunks with no per-chunk branch
+    static constexpr int PLANE = RZ * RYP * XW;
+    static constexpr int LDS = PLANE + 4 * (CPT * NTM - NCHUNK);  // elements per buffer
     static_assert(RY <= RYP, "patch radius too large for the padded slice");
 };
 
@@ -208,13 +212,14 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes)
     const int n = (b >> 31) != 0 ? 0x7fffffff : (int)b;
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
 }
+// off: per-lane byte offset; soff: uniform byte offset (the search offset)
 template <class TS>
-__device__ __forceinline__ float buf_load1(rsrc_t r, unsigned off)
+__device__ __forceinline__ float buf_load1(rsrc_t r, unsigned off, unsigned soff)
 {
     if constexpr (sizeof(TS) == 4) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
     } else {
-        const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
         return (float)__builtin_bit_cast(TS, u);
     }
 }
@@ -244,15 +249,15 @@ struct Stager {
     typedef float chunk[4];
     chunk p0[CPT];
     __device__ __forceinline__ void load(chunk &c, const TS *p) const { load4(p, c); }
-    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off) const
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
         // the builtins return GCC-style vectors; copy them out whole (element
         // subscripts on them are not lowered correctly by this compiler)
         if constexpr (sizeof(TS) == 4) {
-            const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0);
             __builtin_memcpy(c, &q, 16);
         } else {
-            const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+            const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
             TS h[4];
             __builtin_memcpy(h, &q, 8);
 #pragma unroll
@@ -282,9 +287,9 @@ struct Stager<R, _Float16, _Float16> {
     typedef half2v chunk[2];
     chunk p0[CPT];
     __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, 8); }
-    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off) const
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
-        const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
         __builtin_memcpy(&c[0], &q, 8);
     }
     __device__ __forceinline__ void store(const chunk &pn, int c, _Float16 *dst) const
@@ -353,12 +358,12 @@ void filter_mfma(MfmaArgs<TS> a)
     int coff[T::CPT], loff[T::CPT];
 #pragma unroll
     for (int c = 0; c < T::CPT; c++) {
-        int ci = t + c * NTM;
-        if (ci >= T::NCHUNK) ci = 0;
+        const int cs = t + c * NTM;
+        const int ci = cs < T::NCHUNK ? cs : 0;  // spare slots re-read chunk 0
         const int cx = ci % T::CPR, row = ci / T::CPR;
         const int ry = row % T::RY, rz = row / T::RY;
         coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
-        loff[c] = (rz * RYP + ry) * XW + 4 + 4 * cx;
+        loff[c] = cs < T::NCHUNK ? (rz * RYP + ry) * XW + 4 + 4 * cx : T::PLANE + 4 * (cs - T::NCHUNK);
         stg.load(stg.p0[c], Pt + coff[c]);
     }
 
@@ -368,7 +373,7 @@ void filter_mfma(MfmaArgs<TS> a)
     const int z = Z0 + 2 * gz + (q >> 1);
     const int yb = Y0 + 4 * (q & 1);
     // R is read through the tile origin (uniform) plus a 32-bit byte offset
-    // per lane, so the loads take the scalar-base addressing form.
+    // per lane.
     const TS *Rt = a.R + ((long)(Z0 + g.pz) * sp + (long)(Y0 + g.py) * g.nxp + (X0 + g.px));
     const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(yb - Y0) * g.nxp + (x - X0)) * sizeof(TS));
     const unsigned rowbyte = (unsigned)(g.nxp * sizeof(TS));
@@ -386,17 +391,26 @@ void filter_mfma(MfmaArgs<TS> a)
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
 
+    // One buffer resource per volume and tile, starting at the smallest
+    // search offset: a search offset is then the uniform (scalar) part of the
+    // address and the per-lane part stays fixed.  Every address lies inside
+    // the allocation; reads past its end would return zero.
+    const TS *Pb = Pt + a.min_lin;
+    const TS *Rb = Rt + a.min_lin;
+    const rsrc_t rP = make_rsrc(Pb, (long)(a.P_end - Pb) * (long)sizeof(TS));
+    const rsrc_t rR = make_rsrc(Rb, (long)(a.R_end - Rb) * (long)sizeof(TS));
+    auto soff_of = [&](int lin) __attribute__((always_inline)) {
+        return (unsigned)((lin - a.min_lin) * (int)sizeof(TS));
+    };
     typedef typename ST::chunk chunk;
     auto load_chunks = [&](chunk (&pn)[T::CPT], int lin) __attribute__((always_inline)) {
-        const TS *base = Pt + lin;  // uniform
-        const rsrc_t r = make_rsrc(base, (long)(a.P_end - base) * (long)sizeof(TS));
+        const unsigned so = soff_of(lin);
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++) stg.load(pn[c], r, (unsigned)(coff[c] * (int)sizeof(TS)));
+        for (int c = 0; c < T::CPT; c++) stg.load(pn[c], rP, (unsigned)(coff[c] * (int)sizeof(TS)), so);
     };
     auto stage = [&](const chunk (&pn)[T::CPT], TD *buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++)
-            if (T::NCHUNK % NTM == 0 || t + c * NTM < T::NCHUNK) stg.store(pn[c], c, buf + loff[c]);
+        for (int c = 0; c < T::CPT; c++) stg.store(pn[c], c, buf + loff[c]);
     };
 
     // offsets this tile visits: the whole search, or its list
@@ -429,15 +443,16 @@ void filter_mfma(MfmaArgs<TS> a)
 
     for (int k = 0; k < nS; k++) {
         const int lin = lin_of(k);
-        const bool more = (k + 1) < nS;
+        // the next offset's D plane, staged unconditionally (the last
+        // iteration re-stages its own offset into the idle buffer), so the
+        // loop body has no branch
         chunk pn[T::CPT];
-        if (more) load_chunks(pn, lin_of(k + 1));
+        load_chunks(pn, lin_of(k + 1 < nS ? k + 1 : k));
         float rv[4];
         {
-            const TS *rb = Rt + lin;  // uniform
-            const rsrc_t r = make_rsrc(rb, (long)(a.R_end - rb) * (long)sizeof(TS));
+            const unsigned so = soff_of(lin);
 #pragma unroll
-            for (int i = 0; i < 4; i++) rv[i] = buf_load1<TS>(r, rbyte + (unsigned)i * rowbyte);
+            for (int i = 0; i < 4; i++) rv[i] = buf_load1<TS>(rR, rbyte + (unsigned)i * rowbyte, so);
         }
 
         const TD *L = lds[k & 1] + aoff;
@@ -464,12 +479,11 @@ void filter_mfma(MfmaArgs<TS> a)
             __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
             if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
         });
-        // The vector ALU reads the matrix-core result without an interlock;
-        // the compiler's wait-state count missed the branch path that jumps
-        // from the last MFMA straight to the weights (last offset, no
-        // staging), so 16 wait states are spelled out here, ahead of any
-        // reader of acc (more than the 8-pass MFMA needs).
-        asm volatile("s_nop 7\n\ts_nop 7" : "+v"(acc));
+        // The vector ALU reads the matrix-core result without an interlock,
+        // and the compiler counts the wait states it needs.  The loop body is
+        // kept free of branches: with a branch between the last MFMA and the
+        // weights, the count missed the path that skipped the staging and
+        // the last offset's distances were read early.
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const float w = fast_weight<WFN>(acc[i] * mm);
@@ -477,7 +491,7 @@ void filter_mfma(MfmaArgs<TS> a)
             fw[i] = fw[i] + w;
             wmax_bits[i] = __builtin_elementwise_max(wmax_bits[i], __builtin_bit_cast(unsigned, w));
         }
-        if (more) stage(pn, lds[(k + 1) & 1]);
+        stage(pn, lds[(k + 1) & 1]);
         __syncthreads();
     }
 
@@ -583,10 +597,11 @@ bool mfma_bands_r(const int *dxyz, const float *w, int ntap, float (*band)[8])
 template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
-                         const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
-                         const OffsetLists *ol, hipStream_t st)
+                         const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
+                         const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
     MfmaArgs<TS> a;
+    a.min_lin = min_lin;
     a.lists = ol ? ol->lists : nullptr;
     a.counts = ol ? ol->counts : nullptr;
     a.K = ol ? ol->K : 0;
@@ -627,19 +642,19 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const flo
 template <int R, int WFN>
 hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
-                            const SearchEntry *search, int n_search, const TapTable &taps, const FilterGeom &g,
-                            const OffsetLists *ol, hipStream_t st)
+                            const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
+                            const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
         return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                      inv_scale, search, n_search, taps, g, ol, st);
+                                                      inv_scale, search, n_search, min_lin, taps, g, ol, st);
     case FORM_FP16:
         return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                         inv_scale, search, n_search, taps, g, ol, st);
+                                                         inv_scale, search, n_search, min_lin, taps, g, ol, st);
     case FORM_BF16:
         return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, n_search, taps, g, ol, st);
+                                                     inv_scale, search, n_search, min_lin, taps, g, ol, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -660,13 +675,15 @@ bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
                               const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
-                              const TapTable &taps, const FilterGeom &g, hipStream_t st, const OffsetLists *ol)
+                              int min_lin, const TapTable &taps, const FilterGeom &g, hipStream_t st,
+                              const OffsetLists *ol)
 {
+    if (min_lin > 0) return hipErrorInvalidValue;
 #define INLM_MF(RR)                                                                                               \
     return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, taps, g, ol, st)                                 \
+                                                    search, n_search, min_lin, taps, g, ol, st)                                 \
                           : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, taps, g, ol, st)
+                                                    search, n_search, min_lin, taps, g, ol, st)
     switch (rp) {
     case 1: INLM_MF(1);
     case 2: INLM_MF(2);

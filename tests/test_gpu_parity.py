@@ -198,7 +198,8 @@ FAST_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.08, 0.006), "bf16": (1.0, 
 
 
 @pytest.mark.parametrize("form", ["fp32_fast", "fp16", "bf16"])
-@pytest.mark.parametrize("rp,rs", [((1, 1, 1), (3, 3, 3)), ((2, 2, 2), (3, 3, 3)), ((3, 3, 3), (2, 2, 2))])
+@pytest.mark.parametrize("rp,rs", [((1, 1, 1), (3, 3, 3)), ((2, 2, 2), (3, 3, 3)), ((3, 3, 3), (2, 2, 2)),
+                                   ((2, 2, 2), (1, 1, 1))])
 def test_fast_forms_vs_fp64_oracle(gpu_available, form, rp, rs):
     spec = ph.PhantomSpec((40, 30, 20), seed=51, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8, streaks=3,
                           streak_len=(5, 15))
