@@ -49,15 +49,34 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // so rows sharing that key share one B operand.  When the weights depend on
 // dx^2+dy^2+dz^2 alone (no z anisotropy), patterns of equal dy^2+dz^2 carry
 // the same band too and share a B operand class.
+//
+// One MFMA serves one patch row, or two rows of one tap each (half-width 0):
+// a row of one tap needs only the 16 columns of its outputs, so two of them
+// fill the 32-deep K of one MFMA.  The MFMA operations, their B operands
+// (single-row bands and paired taps, one per pattern or per class) and the
+// distinct row distances of the pairs are tabulated here, for the per-pattern
+// (op*) and the per-class (iop*) operand sets.
 struct MRow {
     int dz, dy, j, pat;
 };
+constexpr int kMaxRowOps = 32;
 template <int R>
 struct MRows {
     MRow r[4 * R * R + 4 * R + 1];
     int n, npat, ncls;
     int pdy[kMfmaMaxPatterns], pdz[kMfmaMaxPatterns], pj[kMfmaMaxPatterns];
     int cls[kMfmaMaxPatterns], crep[kMfmaMaxPatterns], ckey[kMfmaMaxPatterns];
+    // MFMA operations: row a, row b (-1: single), index of the pair's row
+    // distance (-1: single)
+    int nops;
+    int opa[kMaxRowOps], opb[kMaxRowOps], opd[kMaxRowOps];
+    int npd;
+    int pdelta[4];  // LDS element distance from row a to row b of a pair
+    // B operands: kind (0 band of a row, 1 paired single taps), pattern;
+    // operand of each operation; per pattern (o*) and per class (i*)
+    int nob, obkind[kMfmaMaxPatterns], obpat[kMfmaMaxPatterns], obop[kMaxRowOps];
+    int nib, ibkind[kMfmaMaxPatterns], ibpat[kMfmaMaxPatterns], ibop[kMaxRowOps];
+    static constexpr int lds_off(const MRow &w) { return (w.dz * RYP + w.dy) * XW; }
     static constexpr bool inside(int x, int y, int z)
     {
         const float rr = (float)R;
@@ -67,7 +86,9 @@ struct MRows {
         e = e + fz * fz;
         return e <= 1.f;
     }
-    constexpr MRows() : r(), n(0), npat(0), ncls(0), pdy(), pdz(), pj(), cls(), crep(), ckey()
+    constexpr MRows()
+        : r(), n(0), npat(0), ncls(0), pdy(), pdz(), pj(), cls(), crep(), ckey(), nops(0), opa(), opb(), opd(),
+          npd(0), pdelta(), nob(0), obkind(), obpat(), obop(), nib(0), ibkind(), ibpat(), ibop()
     {
         for (int z = -R; z <= R; ++z)
             for (int y = -R; y <= R; ++y) {
@@ -98,12 +119,59 @@ struct MRows {
             }
             cls[p] = c;
         }
+        // operations in row order; a one-tap row takes the next unused
+        // one-tap row of its pattern as its partner
+        bool used[4 * R * R + 4 * R + 1] = {};
+        for (int i = 0; i < n; i++) {
+            if (used[i]) continue;
+            used[i] = true;
+            int b = -1;
+            if (r[i].j == 0)
+                for (int k = i + 1; k < n && b < 0; k++)
+                    if (!used[k] && r[k].j == 0 && r[k].pat == r[i].pat) b = k;
+            int d = -1;
+            if (b >= 0) {
+                used[b] = true;
+                const int delta = lds_off(r[b]) - lds_off(r[i]);
+                for (int k = 0; k < npd; k++)
+                    if (pdelta[k] == delta) d = k;
+                if (d < 0) {
+                    d = npd++;
+                    pdelta[d] = delta;
+                }
+            }
+            opa[nops] = i;
+            opb[nops] = b;
+            opd[nops] = d;
+            nops++;
+        }
+        for (int o = 0; o < nops; o++) {
+            const int kind = opb[o] >= 0 ? 1 : 0, p = r[opa[o]].pat;
+            int x = -1, y = -1;
+            for (int k = 0; k < nob; k++)
+                if (obkind[k] == kind && obpat[k] == p) x = k;
+            if (x < 0) {
+                x = nob++;
+                obkind[x] = kind;
+                obpat[x] = p;
+            }
+            for (int k = 0; k < nib; k++)
+                if (ibkind[k] == kind && cls[ibpat[k]] == cls[p]) y = k;
+            if (y < 0) {
+                y = nib++;
+                ibkind[y] = kind;
+                ibpat[y] = crep[cls[p]];
+            }
+            obop[o] = x;
+            ibop[o] = y;
+        }
     }
 };
 template <int R>
 struct MRowsOf {
     static constexpr MRows<R> value{};
-    static_assert(value.npat <= kMfmaMaxPatterns, "too many row patterns");
+    static_assert(value.npat <= kMfmaMaxPatterns && value.nob <= kMfmaMaxPatterns, "too many row patterns");
+    static_assert(value.nops <= kMaxRowOps && value.npd <= 4, "too many MFMA operations");
 };
 
 template <int I, int N, class F>
@@ -161,6 +229,7 @@ struct MfmaArgs {
     const int *counts;
     int K;
     int min_lin;  // smallest linear search offset (<= 0): the tile's buffer ranges start there
+    float one;    // 1.0 (a runtime value the compiler cannot fold)
 };
 
 template <int R>
@@ -182,18 +251,77 @@ __device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
     return (wg & 7) * (nwg >> 3) + (wg >> 3);
 }
 
+// Weight of one distance, from the MFMA accumulator d and a per-lane scale.
+// WFN 0 is the reference's Pade [3/3] with its cutoff (iternlm_cpu.h:125-126)
+// as a degree-7 minimax polynomial in t = a / h + 1 with a = d * m and
+// h = -cutoff / 2, which maps the kept range (cutoff, 0] onto (-1, 1]
+// (largest deviation from the rational form 3.7e-6, weights lie in
+// [0.028, 1]); it needs no reciprocal, a quarter-rate instruction.  The scale
+// is m / h, so t is one fused multiply-add, and the cutoff is t > -1 (a NaN
+// or overflowed distance gives weight zero).  WFN 1 is exp(a) = 2^(a log2 e)
+// with the scale m log2 e.
+// vector instructions per output and offset in the weight step (distance
+// scale, polynomial, cutoff test and select, the three sums)
+constexpr int kWeightValu = 13;
+// per staged chunk of four values (difference, square, clamp; packed for FP16)
+template <class TS>
+constexpr int kStageValu = sizeof(TS) == 2 ? 6 : 12;
+// per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
+// step, product, the sums)
+constexpr int kWeightValuPacked = 16;
+constexpr float kPadePoly[8] = {0.1679810881614685f,  0.30033349990844727f,  0.2661350965499878f,
+                                0.1601015031337738f,  0.06991086900234222f,  0.024921204894781113f,
+                                0.008454217575490475f, 0.002158894669264555f};
+// FP16 storage form: the weights of two outputs per instruction in packed
+// FP16.  u = a / (2h) + 1 = d m / 3.56648 + 1, clamped to [0, 1] (u = 0 at
+// and beyond the cutoff), and the Pade form as a degree-7 minimax polynomial
+// in u with FP16 coefficients, times a step that is 1 for u >= 2^-14 and 0
+// at u = 0.  Evaluated in FP16 it deviates from the rational form by at most
+// 1.1e-3 of the weight (2.6e-4 rms), most of it from rounding u to FP16.
+constexpr _Float16 kPadePolyU[8] = {(_Float16)0.02496337890625f,  (_Float16)0.11541748046875f,
+                                    (_Float16)0.1502685546875f,   (_Float16)0.2890625f,
+                                    (_Float16)-0.055084228515625f, (_Float16)0.625f,
+                                    (_Float16)-0.426025390625f,   (_Float16)0.2763671875f};
+typedef _Float16 half2w __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2w __attribute__((ext_vector_type(2)));
+// the weights of two pairs of outputs, the two polynomial chains in lockstep
+__device__ __forceinline__ void packed_weights(half2w d0, half2w d1, half2w ms, half2w &w0, half2w &w1)
+{
+    const half2w zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
+    half2w u0 = __builtin_elementwise_fma(d0, ms, one), u1 = __builtin_elementwise_fma(d1, ms, one);
+    u0 = __builtin_elementwise_min(__builtin_elementwise_max(u0, zero), one);
+    u1 = __builtin_elementwise_min(__builtin_elementwise_max(u1, zero), one);
+    half2w p0 = {kPadePolyU[7], kPadePolyU[7]}, p1 = p0;
+#pragma unroll
+    for (int j = 6; j >= 0; j--) {
+        const half2w c = {kPadePolyU[j], kPadePolyU[j]};
+        p0 = __builtin_elementwise_fma(p0, u0, c);
+        p1 = __builtin_elementwise_fma(p1, u1, c);
+    }
+    const half2w big = {(_Float16)16384.f, (_Float16)16384.f};
+    half2w s0 = u0 * big, s1 = u1 * big;
+    s0 = __builtin_elementwise_min(__builtin_elementwise_max(s0, zero), one);
+    s1 = __builtin_elementwise_min(__builtin_elementwise_max(s1, zero), one);
+    w0 = p0 * s0;
+    w1 = p1 * s1;
+}
+
 template <int WFN>
-__device__ __forceinline__ float fast_weight(float a)
+__host__ __device__ constexpr float weight_scale_factor()
+{
+    return WFN == 0 ? 2.f / 3.56648f : 1.4426950408889634f;
+}
+template <int WFN>
+__device__ __forceinline__ float fast_weight(float d, float ms)
 {
     if constexpr (WFN == 0) {
-        // Pade [3/3] in Horner form with fused multiply-adds and a hardware
-        // reciprocal: the reference function to a few ulp, not bit-identical.
-        const float num = __builtin_fmaf(__builtin_fmaf(a + 12.f, a, 60.f), a, 120.f);
-        const float den = __builtin_fmaf(__builtin_fmaf(12.f - a, a, -60.f), a, 120.f);
-        const float w = num * __builtin_amdgcn_rcpf(den);
-        return a > kCut ? w : 0.f;
+        const float t = __builtin_fmaf(d, ms, 1.f);
+        float p = kPadePoly[7];
+#pragma unroll
+        for (int j = 6; j >= 0; j--) p = __builtin_fmaf(p, t, kPadePoly[j]);
+        return t > -1.f ? p : 0.f;
     } else {
-        return __builtin_amdgcn_exp2f(a * 1.4426950408889634f);
+        return __builtin_amdgcn_exp2f(d * ms);
     }
 }
 
@@ -212,18 +340,6 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, long bytes)
     const int n = (b >> 31) != 0 ? 0x7fffffff : (int)b;
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, kRsrcWord3);
 }
-// off: per-lane byte offset; soff: uniform byte offset (the search offset)
-template <class TS>
-__device__ __forceinline__ float buf_load1(rsrc_t r, unsigned off, unsigned soff)
-{
-    if constexpr (sizeof(TS) == 4) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
-    } else {
-        const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
-        return (float)__builtin_bit_cast(TS, u);
-    }
-}
-
 template <class TS>
 __device__ __forceinline__ void load4(const TS *p, float (&v)[4])
 {
@@ -246,9 +362,9 @@ __device__ __forceinline__ void load4(const TS *p, float (&v)[4])
 template <int R, class TS, class TD>
 struct Stager {
     static constexpr int CPT = MTile<R>::CPT;
-    typedef float chunk[4];
-    chunk p0[CPT];
-    __device__ __forceinline__ void load(chunk &c, const TS *p) const { load4(p, c); }
+    typedef TS chunk[4];  // stored values, widened to FP32 when staged
+    const chunk *p0;      // this thread's chunks at the tile's own positions (LDS)
+    __device__ __forceinline__ void load(chunk &c, const TS *p) const { __builtin_memcpy(c, p, 4 * sizeof(TS)); }
     __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
         // the builtins return GCC-style vectors; copy them out whole (element
@@ -258,18 +374,17 @@ struct Stager {
             __builtin_memcpy(c, &q, 16);
         } else {
             const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
-            TS h[4];
-            __builtin_memcpy(h, &q, 8);
-#pragma unroll
-            for (int e = 0; e < 4; e++) c[e] = (float)h[e];
+            __builtin_memcpy(c, &q, 8);
         }
     }
     __device__ __forceinline__ void store(const chunk &pn, int c, TD *dst) const
     {
         TD dv[4];
+        chunk b;
+        __builtin_memcpy(b, &p0[c * NTM], sizeof(chunk));
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const float df = pn[e] - p0[c][e];
+            const float df = (float)pn[e] - (float)b[e];
             float dd = df * df;
             if constexpr (std::is_same<TD, _Float16>::value) dd = __builtin_fminf(dd, 65504.f);
             dv[e] = DPlane<TD>::from(dd);
@@ -285,7 +400,7 @@ template <int R>
 struct Stager<R, _Float16, _Float16> {
     static constexpr int CPT = MTile<R>::CPT;
     typedef half2v chunk[2];
-    chunk p0[CPT];
+    const chunk *p0;  // this thread's chunks at the tile's own positions (LDS)
     __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, 8); }
     __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
@@ -295,7 +410,9 @@ struct Stager<R, _Float16, _Float16> {
     __device__ __forceinline__ void store(const chunk &pn, int c, _Float16 *dst) const
     {
         const half2v big = {(_Float16)65504.f, (_Float16)65504.f};
-        half2v d0 = pn[0] - p0[c][0], d1 = pn[1] - p0[c][1];
+        chunk b;
+        __builtin_memcpy(b, &p0[c * NTM], sizeof(chunk));
+        half2v d0 = pn[0] - b[0], d1 = pn[1] - b[1];
         d0 = __builtin_elementwise_min(d0 * d0, big);
         d1 = __builtin_elementwise_min(d1 * d1, big);
         half2v out[2] = {d0, d1};
@@ -304,12 +421,13 @@ struct Stager<R, _Float16, _Float16> {
 };
 
 // 3 workgroups of 8 waves per CU (6 waves per SIMD) fit the LDS for R <= 2;
-// the register budget is capped to match.
-template <int R>
-constexpr int mfma_waves_per_eu() { return R <= 2 ? 6 : 4; }
+// the register budget is capped to match.  FP32 storage holds twice the
+// staging registers and runs 2 workgroups per CU.
+template <int R, class TS>
+constexpr int mfma_waves_per_eu() { return R <= 2 && sizeof(TS) == 2 ? 6 : 4; }
 
 template <int R, class TS, class TD, int WFN, bool PRUNED, bool ISO>
-__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R>(), mfma_waves_per_eu<R>())))
+__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R, TS>(), mfma_waves_per_eu<R, TS>())))
 void filter_mfma(MfmaArgs<TS> a)
 {
     using T = MTile<R>;
@@ -333,19 +451,24 @@ void filter_mfma(MfmaArgs<TS> a)
     // zeros (they meet zero weights in the band, and 0 * NaN would poison).
     for (int i = t; i < T::LDS; i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
 
-    // B operands, one band per pattern (per pattern class when ISO).  Lane l
-    // holds B[k][n] for n = l % 16 and k = 8*(l/16) + i, i.e. input column
-    // x0 - 8 + k, so dx = k - 8 - n.
+    // Band operands, one per pattern (per pattern class when ISO) and kind.
+    // Lane l holds entry k = 8*(l/16) + i of output column n = l % 16.  A
+    // single row's band reads input column x0 - 8 + k, so dx = k - 8 - n; a
+    // pair of one-tap rows reads columns x0 .. x0+15 of its first row for
+    // k < 16 and of its second row for k >= 16, so output n takes k = n and
+    // k = n + 16.
     const int n = lane & 15, q = lane >> 4;
-    constexpr int NB = ISO ? rows.ncls : rows.npat;
+    constexpr int NB = ISO ? rows.nib : rows.nob;
     vec8 bop[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        const int p = ISO ? rows.crep[b] : b;
+        const int p = ISO ? rows.ibpat[b] : rows.obpat[b];
+        const bool pair = (ISO ? rows.ibkind[b] : rows.obkind[b]) != 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int dx = 8 * q + i - 8 - n;
-            const float w = (dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f;
+            const int k = 8 * q + i, dx = k - 8 - n;
+            const float w = pair ? ((k & 15) == n ? a.band[p][R] : 0.f)
+                                 : ((dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f);
             bop[b][i] = DP::from(w);
         }
     }
@@ -354,7 +477,11 @@ void filter_mfma(MfmaArgs<TS> a)
     // slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
     const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
     using ST = Stager<R, TS, TD>;
+    // the chunks at the tile's own positions stay fixed over the offsets and
+    // live in LDS, one slot per (thread, chunk)
+    __shared__ __attribute__((aligned(16))) typename ST::chunk p0s[T::CPT * NTM];
     ST stg;
+    stg.p0 = &p0s[t];
     int coff[T::CPT], loff[T::CPT];
 #pragma unroll
     for (int c = 0; c < T::CPT; c++) {
@@ -364,32 +491,45 @@ void filter_mfma(MfmaArgs<TS> a)
         const int ry = row % T::RY, rz = row / T::RY;
         coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
         loff[c] = cs < T::NCHUNK ? (rz * RYP + ry) * XW + 4 + 4 * cx : T::PLANE + 4 * (cs - T::NCHUNK);
-        stg.load(stg.p0[c], Pt + coff[c]);
+        typename ST::chunk b;
+        stg.load(b, Pt + coff[c]);
+        __builtin_memcpy(&p0s[t + c * NTM], b, sizeof(b));
     }
 
-    // This lane's outputs: C rows m = 4q + i  ->  (z pair member q/2, y 4(q%2)+i).
+    // This lane's outputs.  The MFMA forms the transposed block,
+    // C^T = B^T A^T (band rows times D-plane columns): the A and B operands
+    // of the 16x16x32 MFMA share one lane layout, so the registers are the
+    // same, and lane l then holds row n = l % 16 (slice pair member n/8, row
+    // n%8) at the four consecutive columns 16*xh + 4*q + i.
     const int xh = wave & 1, gz = wave >> 1;
-    const int x = X0 + 16 * xh + n;
-    const int z = Z0 + 2 * gz + (q >> 1);
-    const int yb = Y0 + 4 * (q & 1);
+    const int xb = X0 + 16 * xh + 4 * q;
+    const int z = Z0 + 2 * gz + (n >> 3);
+    const int y = Y0 + (n & 7);
     // R is read through the tile origin (uniform) plus a 32-bit byte offset
-    // per lane.
+    // per lane, four consecutive values per load.
     const TS *Rt = a.R + ((long)(Z0 + g.pz) * sp + (long)(Y0 + g.py) * g.nxp + (X0 + g.px));
-    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(yb - Y0) * g.nxp + (x - X0)) * sizeof(TS));
-    const unsigned rowbyte = (unsigned)(g.nxp * sizeof(TS));
+    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(y - Y0) * g.nxp + (xb - X0)) * sizeof(TS));
     const TS *Rv = reinterpret_cast<const TS *>(reinterpret_cast<const char *>(Rt) + rbyte);
     const int zc = z < g.nz ? z : g.nz - 1;
     const float sig = a.sigma[zc];
     const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
-    // A operand: lane l reads row m = l % 16 (slice pair member m/8, row m%8),
-    // columns 16*xh + 8*q .. +7.
-    const int am = lane & 15;
-    const int aoff = ((2 * gz + (am >> 3) + R) * RYP + ((am & 7) + R)) * XW + 16 * xh + 8 * q;
+    // D operand: lane l reads row n = l % 16, columns 16*xh + 8*q .. +7.
+    const int aoff = ((2 * gz + (n >> 3) + R) * RYP + ((n & 7) + R)) * XW + 16 * xh + 8 * q;
+    // for a pair of one-tap rows: columns x0 + 8 (q % 2) of the first row
+    // (q < 2) or of the second (q >= 2)
+    int poff[rows.npd > 0 ? rows.npd : 1];
+#pragma unroll
+    for (int d = 0; d < rows.npd; d++) poff[d] = aoff + 8 - 16 * (q >> 1) + (q >> 1) * rows.pdelta[d];
 
     // wmax is kept as the bit pattern of a non-negative float: integer max
     // orders those like floats and needs no NaN canonicalisation
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
+    const float ms = mm * weight_scale_factor<WFN>();
+    constexpr bool kPacked = std::is_same<TS, _Float16>::value && WFN == 0;
+    ushort2w wmax2[2] = {{0, 0}, {0, 0}};
+    const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f));
+    const half2w msu = {msu1, msu1};
 
     // One buffer resource per volume and tile, starting at the smallest
     // search offset: a search offset is then the uniform (scalar) part of the
@@ -412,6 +552,47 @@ void filter_mfma(MfmaArgs<TS> a)
 #pragma unroll
         for (int c = 0; c < T::CPT; c++) stg.store(pn[c], c, buf + loff[c]);
     };
+    // R at this lane's four outputs shifted by one search offset, in the
+    // stored type
+    struct RVals {
+        TS v[4];
+    };
+    auto load_r = [&](RVals &rv, unsigned so) __attribute__((always_inline)) {
+        if constexpr (sizeof(TS) == 4) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rR, rbyte, so, 0);
+            __builtin_memcpy(rv.v, &v, 16);
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rR, rbyte, so, 0);
+            __builtin_memcpy(rv.v, &v, 8);
+        }
+    };
+    // the weights of one offset's distances and their sums
+    auto accumulate = [&](const floatx4 &d, const RVals &rv) __attribute__((always_inline)) {
+        if constexpr (kPacked) {
+            half2w w[2];
+            packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
+                           __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float wi = (float)w[i >> 1][i & 1];
+                fv[i] = __builtin_fmaf(wi, (float)rv.v[i], fv[i]);
+                // times a runtime 1.0, so the sum stays one mixed-precision FMA
+                fw[i] = __builtin_fmaf(wi, a.one, fw[i]);
+            }
+            // weights are non-negative: their FP16 bit patterns order like them
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                wmax2[j] = __builtin_elementwise_max(wmax2[j], __builtin_bit_cast(ushort2w, w[j]));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float w = fast_weight<WFN>(d[i], ms);
+                fv[i] = __builtin_fmaf(w, (float)rv.v[i], fv[i]);
+                fw[i] = fw[i] + w;
+                wmax_bits[i] = __builtin_elementwise_max(wmax_bits[i], __builtin_bit_cast(unsigned, w));
+            }
+        }
+    };
 
     // offsets this tile visits: the whole search, or its list
     int nS = a.n_search;
@@ -433,80 +614,111 @@ void filter_mfma(MfmaArgs<TS> a)
         return a.search[k].lin;
     };
 
+    // Software pipeline over the offsets (nS >= 1).  Step k runs the
+    // matrix-core chain of offset k right after the barrier, with the weights
+    // of offset k-1 and the staging arithmetic of offset k+1's D plane (from
+    // chunks loaded one step earlier) interleaved between its MFMAs, then
+    // writes that plane to the idle buffer and loads the chunks of offset
+    // k+2.  The distances of offset k are first read in step k+1, behind its
+    // barrier and its first MFMA (and behind the wait states below the loop
+    // for the last offset).  Steps come in pairs that swap the two distance
+    // registers and the two LDS buffers; when nS is odd the last step
+    // repeats offset nS-1 and its distances are dropped.
+    chunk pn[T::CPT];
     __syncthreads();  // zeroed LDS visible before the first staging
-    {
-        chunk pn[T::CPT];
-        load_chunks(pn, lin_of(0));
-        stage(pn, lds[0]);
-    }
+    auto lin_at = [&](int k) __attribute__((always_inline)) { return lin_of(k < nS ? k : nS - 1); };
+    load_chunks(pn, lin_at(0));
+    stage(pn, lds[0]);
+    load_chunks(pn, lin_at(1));
     __syncthreads();
 
-    for (int k = 0; k < nS; k++) {
-        const int lin = lin_of(k);
-        // the next offset's D plane, staged unconditionally (the last
-        // iteration re-stages its own offset into the idle buffer), so the
-        // loop body has no branch
-        chunk pn[T::CPT];
-        load_chunks(pn, lin_of(k + 1 < nS ? k + 1 : k));
-        float rv[4];
-        {
-            const unsigned so = soff_of(lin);
+    // no offset before the first: an infinite distance has weight zero
+    floatx4 dA = {0.f, 0.f, 0.f, 0.f};
+    floatx4 dB = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+    RVals rA, rB;
 #pragma unroll
-            for (int i = 0; i < 4; i++) rv[i] = buf_load1<TS>(rR, rbyte + (unsigned)i * rowbyte, so);
-        }
-
-        const TD *L = lds[k & 1] + aoff;
+    for (int i = 0; i < 4; i++) rA.v[i] = rB.v[i] = TS(0.f);
+    // The linear offsets of steps kb .. kb+63, one per lane (the search
+    // table or list read once per block, not per step): a block runs steps
+    // kb .. kb+61, which look up to two steps ahead.
+    constexpr int kBlock = 62;
+    int linv = 0, kb = 0;
+    auto step = [&](int k, const TD *rd, TD *st, floatx4 &dout, RVals &rout, const floatx4 &din,
+                    const RVals &rin) __attribute__((always_inline)) {
+        const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
+        const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
+        load_r(rout, soff_of(l0));
+        const TD *L = rd + aoff;
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         // A operands in flight: KD reads ahead of the MFMA that consumes them,
         // so the chain waits on LDS latency once rather than per row
-        constexpr int NR = rows.n;
-        constexpr int KD = NR < 3 ? NR : 3;
-        auto a_of = [&](auto ri) __attribute__((always_inline)) {
-            constexpr MRow rw = MRowsOf<R>::value.r[decltype(ri)::value];
-            return *reinterpret_cast<const vec8 *>(L + (rw.dz * RYP + rw.dy) * XW);
+        constexpr int NR = rows.nops;
+        constexpr int KD = NR < 2 ? NR : 2;
+        auto a_of = [&](auto oi) __attribute__((always_inline)) {
+            constexpr int o = decltype(oi)::value;
+            constexpr const MRows<R> &rw = MRowsOf<R>::value;
+            constexpr int off = MRows<R>::lds_off(rw.r[rw.opa[o]]);
+            if constexpr (rw.opb[o] >= 0)
+                return *reinterpret_cast<const vec8 *>(rd + poff[rw.opd[o]] + off);
+            else
+                return *reinterpret_cast<const vec8 *>(L + off);
         };
         vec8 ab[KD];
         mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = a_of(ri); });
         mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
             constexpr int r = decltype(ri)::value;
-            constexpr MRow rw = MRowsOf<R>::value.r[r];
-            acc = DP::mfma(ab[r % KD], bop[ISO ? MRowsOf<R>::value.cls[rw.pat] : rw.pat], acc);
+            constexpr int b = ISO ? MRowsOf<R>::value.ibop[r] : MRowsOf<R>::value.obop[r];
+            acc = DP::mfma(bop[b], ab[r % KD], acc);
             if constexpr (r + KD < NR) ab[r % KD] = a_of(std::integral_constant<int, r + KD>{});
         });
-        // keep that order: KD reads, then (MFMA, read) pairs, then the tail
+        accumulate(din, rin);
+        stage(pn, st);
+        // order: the R load, KD reads, then per MFMA the next A read and a
+        // share of the vector work (none right after the first MFMA); the
+        // staging writes and the next loads follow the chain
+        constexpr int NV = ((kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu) + kStageValu<TS> * T::CPT + NR - 2) /
+                           (NR - 1);
+        __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
         __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
         mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
             __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
             if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
+            if constexpr (decltype(ri)::value >= 1) __builtin_amdgcn_sched_group_barrier(0x0002, NV, 0);
         });
-        // The vector ALU reads the matrix-core result without an interlock,
-        // and the compiler counts the wait states it needs.  The loop body is
-        // kept free of branches: with a branch between the last MFMA and the
-        // weights, the count missed the path that skipped the staging and
-        // the last offset's distances were read early.
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float w = fast_weight<WFN>(acc[i] * mm);
-            fv[i] = __builtin_fmaf(w, rv[i], fv[i]);
-            fw[i] = fw[i] + w;
-            wmax_bits[i] = __builtin_elementwise_max(wmax_bits[i], __builtin_bit_cast(unsigned, w));
-        }
-        stage(pn, lds[(k + 1) & 1]);
+        // the loads below overwrite the chunks the staging has just read
+        __builtin_amdgcn_sched_barrier(0);
+        load_chunks(pn, l2);
         __syncthreads();
+        dout = acc;
+    };
+    for (kb = 0; kb < nS; kb += kBlock) {
+        linv = lin_at(kb + lane);
+        const int ke = kb + kBlock < nS ? kb + kBlock : nS;
+        for (int k = kb; k < ke; k += 2) {
+            step(k, lds[0], lds[1], dA, rA, dB, rB);
+            step(k + 1, lds[1], lds[0], dB, rB, dA, rA);
+        }
     }
+    // the last offset's distances: wait states for the matrix-core result
+    // (the compiler counts them across the loop exit too; these make sure)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+    if ((nS & 1) == 0) accumulate(dB, rB);
 
     float wmax[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) wmax[i] = __builtin_bit_cast(float, wmax_bits[i]);
-    if (x < g.nx && z < g.nz) {
+    for (int i = 0; i < 4; i++)
+        wmax[i] = kPacked ? (float)__builtin_bit_cast(half2w, wmax2[i >> 1])[i & 1]
+                          : __builtin_bit_cast(float, wmax_bits[i]);
+    if (y < g.ny && z < g.nz) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int y = yb + i;
-            if (y < g.ny) {
+            const int x = xb + i;
+            if (x < g.nx) {
                 const long o = ((long)z * g.ny + y) * g.nx + x;
                 float res = a.raw[o];
                 if (wmax[i] > 0.f) {
-                    const double r0 = (double)(float)Rv[(long)i * g.nxp];
+                    const double r0 = (double)(float)Rv[i];
                     const double v = ((double)fv[i] + (double)wmax[i] * r0) / ((double)fw[i] + (double)wmax[i]);
                     res = (float)(a.shift + v * a.inv_scale);
                 }
@@ -602,6 +814,7 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const flo
 {
     MfmaArgs<TS> a;
     a.min_lin = min_lin;
+    a.one = 1.f;
     a.lists = ol ? ol->lists : nullptr;
     a.counts = ol ? ol->counts : nullptr;
     a.K = ol ? ol->K : 0;
