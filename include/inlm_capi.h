@@ -162,6 +162,19 @@ INLM_API inlm_status inlm_estimate_noise(inlm_handle *h, int32_t which, const in
  * (maxiterations*nz floats). */
 INLM_API inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, float *sigma_log);
 
+/* Out-of-core schedule: inlm_denoise over z-slabs of at most slab_slices
+ * output slices, each uploaded with the filter's z reach on both sides, for
+ * volumes whose buffers do not fit one GPU together.  Replaces the host- and
+ * file-staged block drivers Run_GaussianNoise_GPUBlocks
+ * (iternlm_gpu.cu:1729-2115) and the CPU block mode (iternlm_cpu.cpp:403-577),
+ * with sigma indexed by global z (the reference indexes it by block-local
+ * z, iternlm_gpu.cu:121).  raw, out: host, nx*ny*nz floats; sigma_log NULL
+ * or maxiterations*nz floats as in inlm_denoise.  The FP32 form gives the
+ * whole-volume result bit for bit; the 16-bit forms normalise each slab
+ * by its own moments. */
+INLM_API inlm_status inlm_denoise_slabs(const inlm_params *p, const int32_t shape[3], int32_t device,
+                                        const float *raw, float *out, int32_t slab_slices, float *sigma_log);
+
 /* Host-only query of the tables a handle with these parameters would use on
  * a volume of `shape`: search offsets and patch taps as (dx,dy,dz) triples in
  * the reference order (setup_searchspace / setup_patchspace,

@@ -35,7 +35,7 @@ EXPORTS = [
     # resume, volume I/O and the program's output contract
     "inlm_set_result", "inlm_tiff_stack_shape", "inlm_tiff_read_stack", "inlm_tiff_write_stack",
     "inlm_noise_csv_write", "inlm_noise_csv_read", "inlm_iteration_on_disk", "inlm_default_io_options",
-    "inlm_denoise_files",
+    "inlm_denoise_files", "inlm_denoise_slabs",
 ]
 
 
@@ -145,6 +145,8 @@ def lib():
         L.inlm_default_io_options.restype = None
         L.inlm_denoise_files.argtypes = [C.POINTER(InlmParams), C.c_int32, C.c_char_p, C.c_char_p,
                                          C.POINTER(InlmIoOptions)]
+        L.inlm_denoise_slabs.argtypes = [C.POINTER(InlmParams), C.POINTER(C.c_int32), C.c_int32, f32p, f32p,
+                                         C.c_int32, f32p]
         _LIB = L
     return _LIB
 

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinlm.so")
 
-SOURCES = ["inlm_kernels.hip", "inlm_mfma.hip", "inlm_learn.hip", "inlm_capi.cpp", "inlm_tables.cpp", "inlm_io.cpp"]
+SOURCES = ["inlm_kernels.hip", "inlm_mfma.hip", "inlm_learn.hip", "inlm_capi.cpp", "inlm_tables.cpp", "inlm_io.cpp",
+           "inlm_slabs.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
@@ -63,6 +64,25 @@ def build_facade_test(verbose: bool = False) -> str:
     return FACADE_BIN
 
 
+CLI_SRC = os.path.join(CSRC, "inlm_main.cpp")
+CLI_BIN = os.path.join(HERE, "inlm")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """Compiles the command-line program (the reference's main.cpp flags over
+    inlm_denoise_files) against libinlm.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), CLI_SRC, "-L" + HERE, "-linlm",
+           "-Wl,-rpath," + HERE, "-o", CLI_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the command-line program")
+    return CLI_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_facade_test(verbose=True))
+    print(build_cli(verbose=True))

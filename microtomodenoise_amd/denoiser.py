@@ -274,3 +274,21 @@ def denoise(instack: np.ndarray, params: DenoiseParameters, device: int = 0):
     finally:
         g.free_device()
     return out, log[: params.maxiterations]
+
+
+def denoise_slabs(instack: np.ndarray, params: DenoiseParameters, slab_slices: int, device: int = 0):
+    """The same schedule over z-slabs of at most `slab_slices` output slices
+    (inlm_denoise_slabs), for volumes too large for one GPU at once: returns
+    (result, per-iteration sigma lists).  Replaces the reference's block
+    drivers (Run_GaussianNoise_GPUBlocks, iternlm_gpu.cu:1729-2115)."""
+    raw = _f32(instack)
+    nz, ny, nx = raw.shape
+    out = np.empty_like(raw)
+    log = np.zeros((max(params.maxiterations, 1), nz), np.float32)
+    sh = np.array([nx, ny, nz], np.int32)
+    L = capi.lib()
+    st = L.inlm_denoise_slabs(C.byref(params.to_c()), capi.ptr(sh, C.c_int32), device, capi.ptr(raw, C.c_float),
+                              capi.ptr(out, C.c_float), int(slab_slices), capi.ptr(log, C.c_float))
+    capi.check(st)
+    return out, log[: params.maxiterations]
+
