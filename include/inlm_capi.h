@@ -321,6 +321,35 @@ INLM_API void inlm_default_io_options(inlm_io_options *o);
 INLM_API inlm_status inlm_denoise_files(const inlm_params *p, int32_t device, const char *inpath,
                                         const char *outpath, const inlm_io_options *o);
 
+/* ---- colour path (Color/color_denoising.h, iternlm_cpu.cpp:932-1075) ----- */
+typedef struct inlm_rgb_options {
+    int32_t independent_channels; /* --independent: a weight per channel (default: one joint weight) */
+    int32_t stabilize_variance;   /* Anscombe transform of the channels (default 1; --nopoisson: 0) */
+    int32_t average_channelsigma; /* one noise level for all channels (default 1; --noaverage: 0) */
+    int32_t reserved[5];
+} inlm_rgb_options;
+INLM_API void inlm_default_rgb_options(inlm_rgb_options *c);
+
+/* One 2D RGB image, replacing color_denoise::run_colorimage_denoising for an
+ * image in memory: in and out hold the R, G and B planes of nx*ny floats
+ * each (shape = {nx, ny}); the z radii are ignored (2D) and the noise level
+ * of each channel comes from 2048 samples.  The result is returned after the
+ * inverse Anscombe transform, unquantised.  sigma_log: NULL or
+ * 3*maxiterations floats, the channel noise levels each iteration used. */
+INLM_API inlm_status inlm_denoise_rgb(const inlm_params *p, const inlm_rgb_options *c, const int32_t shape[2],
+                                      int32_t device, const float *in, float *out, float *sigma_log);
+/* RGB TIFF (8 or 16 bits, 3 or 4 samples per pixel) to three float planes,
+ * and three float planes to an 8-bit RGB TIFF (values capped at 255 and
+ * truncated, as color_denoising.h:72-77). */
+INLM_API inlm_status inlm_rgb_tiff_shape(const char *file, int32_t shape[2]);
+INLM_API inlm_status inlm_rgb_tiff_read(const char *file, float *planes);
+INLM_API inlm_status inlm_rgb_tiff_write(const char *file, const float *planes, const int32_t shape[2]);
+/* The reference program's colour mode (main.cpp:254-257): every RGB TIFF in
+ * inpath (or the one file) to outpath/<name>_denoised.tif, and, unless
+ * cleanup, every iteration to outpath/denoise_iterations/<name>_iterNN.tif. */
+INLM_API inlm_status inlm_denoise_rgb_files(const inlm_params *p, const inlm_rgb_options *c, int32_t device,
+                                            const char *inpath, const char *outpath, const inlm_io_options *o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,8 +35,19 @@ EXPORTS = [
     # resume, volume I/O and the program's output contract
     "inlm_set_result", "inlm_tiff_stack_shape", "inlm_tiff_read_stack", "inlm_tiff_write_stack",
     "inlm_noise_csv_write", "inlm_noise_csv_read", "inlm_iteration_on_disk", "inlm_default_io_options",
-    "inlm_denoise_files", "inlm_denoise_slabs",
+    "inlm_denoise_files", "inlm_denoise_slabs", "inlm_default_rgb_options", "inlm_denoise_rgb",
+    "inlm_rgb_tiff_shape", "inlm_rgb_tiff_read", "inlm_rgb_tiff_write", "inlm_denoise_rgb_files",
 ]
+
+
+class InlmRgbOptions(C.Structure):
+    """Mirror of ``inlm_rgb_options`` (include/inlm_capi.h)."""
+    _fields_ = [
+        ("independent_channels", C.c_int32),
+        ("stabilize_variance", C.c_int32),
+        ("average_channelsigma", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
 
 
 class InlmIoOptions(C.Structure):
@@ -145,6 +156,15 @@ def lib():
         L.inlm_default_io_options.restype = None
         L.inlm_denoise_files.argtypes = [C.POINTER(InlmParams), C.c_int32, C.c_char_p, C.c_char_p,
                                          C.POINTER(InlmIoOptions)]
+        L.inlm_default_rgb_options.argtypes = [C.POINTER(InlmRgbOptions)]
+        L.inlm_default_rgb_options.restype = None
+        L.inlm_denoise_rgb.argtypes = [C.POINTER(InlmParams), C.POINTER(InlmRgbOptions), C.POINTER(C.c_int32),
+                                       C.c_int32, f32p, f32p, f32p]
+        L.inlm_rgb_tiff_shape.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
+        L.inlm_rgb_tiff_read.argtypes = [C.c_char_p, f32p]
+        L.inlm_rgb_tiff_write.argtypes = [C.c_char_p, f32p, C.POINTER(C.c_int32)]
+        L.inlm_denoise_rgb_files.argtypes = [C.POINTER(InlmParams), C.POINTER(InlmRgbOptions), C.c_int32, C.c_char_p,
+                                             C.c_char_p, C.POINTER(InlmIoOptions)]
         L.inlm_denoise_slabs.argtypes = [C.POINTER(InlmParams), C.POINTER(C.c_int32), C.c_int32, f32p, f32p,
                                          C.c_int32, f32p]
         _LIB = L

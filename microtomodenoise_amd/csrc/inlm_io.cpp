@@ -231,6 +231,41 @@ public:
         return true;
     }
 
+    // an RGB image (3 samples per pixel, a 4th alpha sample ignored, as
+    // hdcommunication.cpp:708-711), 8 or 16 bits, interleaved; out holds
+    // the R, G and B planes of width*height floats each
+    bool check_rgb(const TiffImage &im)
+    {
+        if (im.width == 0 || im.height == 0) return fail("image without width or height");
+        if (im.tiled) return fail("tiled TIFF is not supported");
+        if (im.compression != 1) return fail("compressed TIFF is not supported");
+        if (im.spp != 3 && im.spp != 4) return fail("an RGB image needs 3 or 4 samples per pixel");
+        if (im.planar != 1) return fail("only interleaved (contiguous) RGB samples are supported");
+        if (im.bps != 8 && im.bps != 16) return fail("only 8- and 16-bit RGB samples are supported");
+        if (im.offsets.empty() || im.offsets.size() != im.counts.size()) return fail("missing strip tables");
+        return true;
+    }
+    bool read_rgb(const TiffImage &im, float *out)
+    {
+        const size_t bpp = im.bps / 8, spp = im.spp;
+        const uint64_t n = (uint64_t)im.width * im.height, bytes = n * spp * bpp;
+        std::vector<uint8_t> raw(bytes);
+        const uint64_t rps = std::min<uint64_t>(im.rows_per_strip, im.height);
+        uint64_t done = 0;
+        for (size_t s = 0; s < im.offsets.size() && done < bytes; s++) {
+            const uint64_t want = std::min<uint64_t>(rps * im.width * spp * bpp, bytes - done);
+            if (!read_at(im.offsets[s], raw.data() + done, want)) return fail("strip outside the file");
+            done += want;
+        }
+        if (done < bytes) return fail("strips hold fewer rows than the image");
+        for (uint64_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) {
+                const uint8_t *q = &raw[(i * spp + c) * bpp];
+                out[c * n + i] = bpp == 1 ? (float)q[0] : (float)u16(q);
+            }
+        return true;
+    }
+
 private:
     std::string path_;
     FILE *f_ = nullptr;
@@ -433,6 +468,104 @@ bool write_slice(const std::string &file, const float *img, int width, int heigh
     if (!f) return false;
     const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
     return std::fclose(f) == 0 && ok;
+}
+
+// An 8-bit RGB image as Save2DTifImage_RGB writes it (hdcommunication.cpp:
+// 1487-1560): interleaved samples, rows per strip 2048 / width, no
+// compression.  rgb holds width*height*3 bytes.
+bool write_rgb(const std::string &file, const uint8_t *rgb, int width, int height)
+{
+    const uint32_t rps = std::max(1, 2048 / width);
+    const uint32_t nstrips = (height + rps - 1) / rps;
+    const uint64_t data_bytes = (uint64_t)width * height * 3;
+    if (data_bytes + 4096 + 8ull * nstrips > 0xffffffffull) return false;
+    std::vector<uint8_t> b;
+    b.reserve(data_bytes + 256 + 8 * nstrips);
+    b.push_back('I');
+    b.push_back('I');
+    put16(b, 42);
+    const uint32_t ifd_off = (uint32_t)(8 + data_bytes + (data_bytes & 1));
+    put32(b, ifd_off);
+    b.insert(b.end(), rgb, rgb + data_bytes);
+    if (data_bytes & 1) b.push_back(0);
+    const int nent = 11;
+    const uint32_t after_ifd = ifd_off + 2 + 12 * nent + 4;
+    const uint32_t bps_tab = after_ifd, fmt_tab = after_ifd + 6;
+    const uint32_t off_tab = after_ifd + 12, cnt_tab = off_tab + (nstrips > 1 ? 4 * nstrips : 0);
+    auto entry = [&](uint16_t tag, uint16_t type, uint32_t count, uint32_t value) {
+        put16(b, tag);
+        put16(b, type);
+        put32(b, count);
+        if (type == 3 && count == 1) {
+            put16(b, (uint16_t)value);
+            put16(b, 0);
+        } else
+            put32(b, value);
+    };
+    put16(b, nent);
+    entry(256, 4, 1, (uint32_t)width);
+    entry(257, 4, 1, (uint32_t)height);
+    entry(258, 3, 3, bps_tab);  // 8, 8, 8
+    entry(259, 3, 1, 1);        // no compression
+    entry(262, 3, 1, 2);        // RGB
+    entry(273, 4, nstrips, nstrips > 1 ? off_tab : 8);
+    entry(277, 3, 1, 3);
+    entry(278, 4, 1, rps);
+    entry(279, 4, nstrips, nstrips > 1 ? cnt_tab : (uint32_t)data_bytes);
+    entry(284, 3, 1, 1);         // contiguous
+    entry(339, 3, 3, fmt_tab);   // unsigned integer x 3
+    put32(b, 0);
+    put16(b, 8);
+    put16(b, 8);
+    put16(b, 8);
+    put16(b, 1);
+    put16(b, 1);
+    put16(b, 1);
+    if (nstrips > 1) {
+        const uint64_t strip_bytes = (uint64_t)rps * width * 3;
+        for (uint32_t s = 0; s < nstrips; s++) put32(b, (uint32_t)(8 + s * strip_bytes));
+        for (uint32_t s = 0; s < nstrips; s++)
+            put32(b, (uint32_t)std::min<uint64_t>(strip_bytes, data_bytes - s * strip_bytes));
+    }
+    FILE *f = std::fopen(file.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(b.data(), 1, b.size(), f) == b.size();
+    return std::fclose(f) == 0 && ok;
+}
+
+// float channels to the 8-bit output of color_denoising.h:72-77: the value
+// capped at 255 and truncated toward zero (the channels are non-negative
+// there; a negative value is written as 0 here)
+void to_rgb8(const float *planes, size_t n, uint8_t *rgb)
+{
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) {
+            const float v = std::min(255.f, planes[c * n + i]);
+            rgb[3 * i + c] = v > 0.f ? (uint8_t)v : (uint8_t)0;
+        }
+}
+
+bool read_rgb_file(const std::string &file, int shape[2], std::vector<float> *planes, std::string &err)
+{
+    TiffFile f(file);
+    if (!f.ok()) {
+        err = file + ": cannot open";
+        return false;
+    }
+    std::vector<TiffImage> ifds;
+    if (!f.parse(ifds) || !f.check_rgb(ifds[0])) {
+        err = f.err;
+        return false;
+    }
+    shape[0] = (int)ifds[0].width;
+    shape[1] = (int)ifds[0].height;
+    if (!planes) return true;
+    planes->resize((size_t)3 * shape[0] * shape[1]);
+    if (!f.read_rgb(ifds[0], planes->data())) {
+        err = f.err;
+        return false;
+    }
+    return true;
 }
 
 bool write_stack(const std::string &dir, const std::string &name, const float *vol, int nx, int ny, int nz, int z0,
@@ -751,6 +884,94 @@ inlm_status inlm_denoise_files(const inlm_params *pin, int32_t device, const cha
         lf << "    search space: (" << p.radius_searchspace[0] << "," << p.radius_searchspace[1] << ","
            << p.radius_searchspace[2] << ")\n";
         lf << "-------------------------------------------------------------------------\n\n";
+    }
+    return INLM_OK;
+}
+
+inlm_status inlm_rgb_tiff_shape(const char *file, int32_t shape[2])
+{
+    if (!file || !shape) return io_fail("null argument");
+    int sh[2];
+    std::string err;
+    if (!read_rgb_file(file, sh, nullptr, err)) return io_fail(err);
+    shape[0] = sh[0];
+    shape[1] = sh[1];
+    return INLM_OK;
+}
+
+inlm_status inlm_rgb_tiff_read(const char *file, float *planes)
+{
+    if (!file || !planes) return io_fail("null argument");
+    int sh[2];
+    std::vector<float> v;
+    std::string err;
+    if (!read_rgb_file(file, sh, &v, err)) return io_fail(err);
+    std::memcpy(planes, v.data(), v.size() * sizeof(float));
+    return INLM_OK;
+}
+
+inlm_status inlm_rgb_tiff_write(const char *file, const float *planes, const int32_t shape[2])
+{
+    if (!file || !planes || !shape || shape[0] < 1 || shape[1] < 1) return io_fail("bad argument");
+    const size_t n = (size_t)shape[0] * shape[1];
+    std::vector<uint8_t> rgb(3 * n);
+    to_rgb8(planes, n, rgb.data());
+    if (!write_rgb(file, rgb.data(), shape[0], shape[1])) return io_fail(std::string(file) + ": cannot write");
+    return INLM_OK;
+}
+
+inlm_status inlm_denoise_rgb_files(const inlm_params *p, const inlm_rgb_options *copt, int32_t device,
+                                   const char *inpath_c, const char *outpath_c, const inlm_io_options *oin)
+{
+    if (!p || !inpath_c) return io_fail("null argument");
+    inlm_io_options o;
+    if (oin) o = *oin;
+    else inlm_default_io_options(&o);
+    inlm_rgb_options c;
+    if (copt) c = *copt;
+    else inlm_default_rgb_options(&c);
+    const std::string inpath = inpath_c;
+    // main.cpp:213-218, as for volumes
+    std::string rootpath = inpath.substr(0, inpath.rfind('/', inpath.size() >= 2 ? inpath.size() - 2 : 0) + 1);
+    std::string outpath = outpath_c ? outpath_c : "";
+    if (outpath.empty()) outpath = rootpath + "/denoised/";
+    std::vector<std::string> files;
+    if (dir_exists(inpath)) {
+        if (!list_tifs(inpath, files) || files.empty()) return io_fail(inpath + ": no TIFF images");
+    } else if (file_exists(inpath))
+        files.push_back(inpath);
+    else
+        return io_fail(inpath + ": no such file or folder");
+    if (!make_path(outpath)) return io_fail(outpath + ": cannot create");
+    const std::string iterdir = outpath + "/denoise_iterations/";
+    // color_denoising.h:19-150, one image after the other
+    for (const std::string &file : files) {
+        const size_t slash = file.rfind('/'), dot = file.rfind('.');
+        const std::string name = file.substr(slash + 1, dot - slash - 1);
+        int sh[2];
+        std::vector<float> planes;
+        std::string err;
+        if (!read_rgb_file(file, sh, &planes, err)) return io_fail(err);
+        const size_t n = (size_t)sh[0] * sh[1];
+        std::vector<float> out(3 * n);
+        std::vector<uint8_t> rgb(3 * n);
+        const int32_t shape[2] = {sh[0], sh[1]};
+        if (o.verbose) std::printf("image: %s (%d x %d)\n", name.c_str(), sh[0], sh[1]);
+        auto on_iteration = [&](int iter, float *res) -> bool {
+            if (o.cleanup) return true;
+            // written through the inverse transform, then transformed back:
+            // that rounding reaches the next iteration, as in the reference
+            if (c.stabilize_variance) inlm::inverse_anscombe(res, 3 * n);
+            to_rgb8(res, n, rgb.data());
+            if (c.stabilize_variance) inlm::anscombe(res, 3 * n);
+            if (!make_path(iterdir)) return false;
+            return write_rgb(iterdir + name + "_iter" + zfill(iter, 2) + ".tif", rgb.data(), sh[0], sh[1]);
+        };
+        const inlm_status st = inlm::denoise_rgb(p, &c, shape, device, planes.data(), out.data(), nullptr, on_iteration);
+        if (st != INLM_OK) return st;
+        to_rgb8(out.data(), n, rgb.data());
+        if (!write_rgb(outpath + "/" + name + "_denoised.tif", rgb.data(), sh[0], sh[1]))
+            return io_fail(outpath + ": cannot write " + name + "_denoised.tif");
     }
     return INLM_OK;
 }

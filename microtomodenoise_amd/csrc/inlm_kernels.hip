@@ -461,6 +461,101 @@ __global__ __launch_bounds__(NT) void filter_pervoxel_fp32(VoxelArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Three-channel filter of the colour path (iternlm_cpu.cpp:932-1075): one
+// thread per output pixel, the three channels' distances in patch order,
+// each scaled by its channel's multiplier.  Joint weights (the default) take
+// the mean of the three scaled distances and weight all channels alike,
+// the centre counted once with the largest weight; independent channels
+// weigh each channel by its own distance and, as the reference does, leave
+// the centre out.  Every product and sum rounds as the reference's float code
+// does (no contraction, IEEE division).
+// ---------------------------------------------------------------------------
+struct RgbArgs {
+    const float *R[3];  // padded originals
+    const float *P[3];  // padded previous iterates
+    float *out[3];      // unpadded results
+    const SearchEntry *search;
+    int n_search;
+    float mult[3];  // -1 / (sigma_c^2 beta)
+    int independent;
+    FilterGeom g;
+    TapTable taps;
+};
+
+template <int WFN>
+__global__ __launch_bounds__(NT) void filter_rgb_fp32(RgbArgs a)
+{
+    const FilterGeom g = a.g;
+    const long nvox = (long)g.nx * g.ny * g.nz;
+    const long idx = (long)blockIdx.x * NT + threadIdx.x;
+    if (idx >= nvox) return;
+    const int x = (int)(idx % g.nx);
+    const int y = (int)((idx / g.nx) % g.ny);
+    const int z = (int)(idx / ((long)g.nx * g.ny));
+    const long sp = (long)g.nxp * g.nyp;
+    const long i0 = (long)(z + g.pz) * sp + (long)(y + g.py) * g.nxp + (x + g.px);
+    const float *P0 = a.P[0], *P1 = a.P[1], *P2 = a.P[2];
+    float fv0 = 0.f, fv1 = 0.f, fv2 = 0.f, fw0 = 0.f, fw1 = 0.f, fw2 = 0.f, wm0 = 0.f, wm1 = 0.f, wm2 = 0.f;
+    for (int k = 0; k < a.n_search; k++) {
+        const long i1 = i0 + a.search[k].lin;
+        float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+        for (int p = 0; p < a.taps.n; p++) {
+            const int pp = a.taps.lin[p];
+            const float dw = a.taps.w[p];
+            float t = P0[i1 + pp] - P0[i0 + pp];
+            d0 = d0 + (t * t) * dw;
+            t = P1[i1 + pp] - P1[i0 + pp];
+            d1 = d1 + (t * t) * dw;
+            t = P2[i1 + pp] - P2[i0 + pp];
+            d2 = d2 + (t * t) * dw;
+        }
+        d0 = d0 * a.mult[0];
+        d1 = d1 * a.mult[1];
+        d2 = d2 * a.mult[2];
+        if (!a.independent) {
+            const float ds = ((d0 + d1) + d2) / 3.f;
+            const float w = weight_of<WFN>(ds);
+            fv0 = fv0 + w * a.R[0][i1];
+            fv1 = fv1 + w * a.R[1][i1];
+            fv2 = fv2 + w * a.R[2][i1];
+            fw0 = fw0 + w;
+            wm0 = w > wm0 ? w : wm0;
+        } else {
+            const float w0 = weight_of<WFN>(d0), w1 = weight_of<WFN>(d1), w2 = weight_of<WFN>(d2);
+            fv0 = fv0 + w0 * a.R[0][i1];
+            fv1 = fv1 + w1 * a.R[1][i1];
+            fv2 = fv2 + w2 * a.R[2][i1];
+            fw0 = fw0 + w0;
+            fw1 = fw1 + w1;
+            fw2 = fw2 + w2;
+            wm0 = w0 > wm0 ? w0 : wm0;
+            wm1 = w1 > wm1 ? w1 : wm1;
+            wm2 = w2 > wm2 ? w2 : wm2;
+        }
+    }
+    const float r0 = a.R[0][i0], r1 = a.R[1][i0], r2 = a.R[2][i0];
+    float o0 = r0, o1 = r1, o2 = r2;
+    if (!a.independent) {
+        if (wm0 > 0.f) {
+            fv0 = fv0 + wm0 * r0;
+            fv1 = fv1 + wm0 * r1;
+            fv2 = fv2 + wm0 * r2;
+            fw0 = fw0 + wm0;
+            o0 = fv0 / fw0;
+            o1 = fv1 / fw0;
+            o2 = fv2 / fw0;
+        }
+    } else {
+        if (wm0 > 0.f) o0 = fv0 / fw0;
+        if (wm1 > 0.f) o1 = fv1 / fw1;
+        if (wm2 > 0.f) o2 = fv2 / fw2;
+    }
+    a.out[0][idx] = o0;
+    a.out[1][idx] = o1;
+    a.out[2][idx] = o2;
+}
+
+// ---------------------------------------------------------------------------
 // Reflective padding (iternlm_prepare.cpp:6-48): mirror without repeating the
 // edge, iterated until in range.
 // ---------------------------------------------------------------------------
@@ -649,6 +744,29 @@ static hipError_t launch_pervoxel(const float *Rv, const float *P, float *out, c
     a.taps = taps;
     const long n = (long)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL((filter_pervoxel_fp32<WFN>), dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_rgb(int weight_fn, const float *const R[3], const float *const P[3], float *const out[3],
+                             const float mult[3], int independent, const SearchEntry *search, int n_search,
+                             const TapTable &taps, const FilterGeom &g, hipStream_t st)
+{
+    RgbArgs a;
+    for (int c = 0; c < 3; c++) {
+        a.R[c] = R[c];
+        a.P[c] = P[c];
+        a.out[c] = out[c];
+        a.mult[c] = mult[c];
+    }
+    a.search = search;
+    a.n_search = n_search;
+    a.independent = independent;
+    a.g = g;
+    a.taps = taps;
+    const long n = (long)g.nx * g.ny * g.nz;
+    const dim3 grid((unsigned)((n + NT - 1) / NT)), block(NT);
+    if (weight_fn == 0) hipLaunchKernelGGL((filter_rgb_fp32<0>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((filter_rgb_fp32<1>), grid, block, 0, st, a);
     return hipGetLastError();
 }
 

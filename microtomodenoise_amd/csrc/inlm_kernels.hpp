@@ -45,6 +45,12 @@ hipError_t launch_filter(int form, int rp, int weight_fn, const float *R, const 
                          const float *sigma, float beta, const SearchEntry *search, int n_search,
                          const TapTable &taps, const FilterGeom &g, hipStream_t st, float *mass = nullptr);
 
+// Three-channel filter of the colour path (iternlm_cpu.cpp:932-1075) on
+// padded FP32 channels; mult[c] = -1 / (sigma_c^2 beta).
+hipError_t launch_filter_rgb(int weight_fn, const float *const R[3], const float *const P[3], float *const out[3],
+                             const float mult[3], int independent, const SearchEntry *search, int n_search,
+                             const TapTable &taps, const FilterGeom &g, hipStream_t st);
+
 // Whether the tiled kernel for isotropic radius rp uses exactly the given tap
 // order (dx,dy,dz per tap, centre first) and whether all taps it groups into
 // one weight class carry the same weight.

@@ -4,10 +4,10 @@
 //
 // Flags keep the reference's names and meanings; the run itself is
 // inlm_denoise_files (noise estimate, iterations, output folders, resume,
-// cleanup, log file, as main.cpp:231-591).  Flags that select parts this
-// build leaves out (RGB input, several GPUs, CPU threading and unrolling)
-// are accepted where the reference accepts them and reported; RGB input is
-// refused.  Added flags: -precision, -weight, -seed, -device (same as
+// cleanup, log file, as main.cpp:231-591), or inlm_denoise_rgb_files with
+// --rgb (the colour path, main.cpp:254-257).  Flags that select parts this
+// build leaves out (several GPUs, CPU threading and unrolling) are accepted
+// where the reference accepts them and reported.  Added flags: -precision, -weight, -seed, -device (same as
 // -gpu0) and --print-params, which prints the parsed settings and exits
 // without touching a GPU.
 #include <cstdio>
@@ -23,6 +23,7 @@ const char *kUsage =
     "usage: inlm <input folder or TIFF stack> [options]\n"
     "  data:     -i/-input PATH  -o/-output PATH  -r/-resume true|false  --resume\n"
     "            -fs/-firstslice N  -ls/-lastslice N (inclusive)  --16bit  --cleanup  -aniso/-anisotropy F\n"
+    "            --rgb/--color (2D RGB images)  --independent  --nopoisson  --noaverage\n"
     "  denoise:  -maxiter/-iter/-iterations N  -slices/-nslices N  -a/-alpha F\n"
     "            -search N  -search0/1/2 N  -patch N  -patch0/1/2 N\n"
     "  noise:    -noise/-noisemode global|z-adaptive|manual  -sigma/-s F  -sigma0/-s0 F  -sigma1/-s1 F\n"
@@ -57,7 +58,9 @@ int main(int argc, char **argv)
     inlm_default_io_options(&o);
     std::string inpath, outpath;
     int device = 0, ngpus = 1;
-    bool print_params = false;
+    bool print_params = false, rgb = false;
+    inlm_rgb_options c;
+    inlm_default_rgb_options(&c);
 
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
@@ -86,7 +89,7 @@ int main(int argc, char **argv)
         else if (is(a, "-ls") || is(a, "-lastslice")) { NEED(a); o.lastslice = std::atoi(v); }
         else if (is(a, "--16bit")) o.save_16bit = 1;
         else if (is(a, "--cleanup")) o.cleanup = 1;
-        else if (is(a, "--color") || is(a, "--rgb")) return fail("RGB input is not supported by this build", nullptr);
+        else if (is(a, "--color") || is(a, "--rgb")) rgb = true;
         else if (is(a, "-aniso") || is(a, "-anisotropy")) { NEED(a); p.z_anisotropy = (float)std::atof(v); }
         // denoising
         else if (is(a, "-maxiter") || is(a, "-iter") || is(a, "-iterations")) { NEED(a); p.maxiterations = std::atoi(v); }
@@ -106,9 +109,9 @@ int main(int argc, char **argv)
         else if (is(a, "-patch0")) { NEED(a); p.radius_patchspace[0] = std::atoi(v); }
         else if (is(a, "-patch1")) { NEED(a); p.radius_patchspace[1] = std::atoi(v); }
         else if (is(a, "-patch2")) { NEED(a); p.radius_patchspace[2] = std::atoi(v); }
-        else if (is(a, "--independent") || is(a, "--nopoisson") || is(a, "--noaverage")) {
-            // colour-pipeline switches: no effect without RGB input
-        }
+        else if (is(a, "--independent")) c.independent_channels = 1;
+        else if (is(a, "--nopoisson")) c.stabilize_variance = 0;
+        else if (is(a, "--noaverage")) c.average_channelsigma = 0;
         // hardware
         else if (is(a, "-cpus") || is(a, "-n_cpus") || is(a, "-threads")) { NEED(a); }
         else if (is(a, "-gpus") || is(a, "-n_gpus") || is(a, "-ngpus")) { NEED(a); ngpus = std::atoi(v); }
@@ -185,6 +188,8 @@ int main(int argc, char **argv)
                     (unsigned long long)p.sample_seed);
         std::printf("resume=%d\nfirstslice=%d\nlastslice=%d\nsave_16bit=%d\ncleanup=%d\nverbose=%d\ndevice=%d\n",
                     o.resume, o.firstslice, o.lastslice, o.save_16bit, o.cleanup, o.verbose, device);
+        std::printf("rgb=%d\nindependent_channels=%d\nstabilize_variance=%d\naverage_channelsigma=%d\n", rgb ? 1 : 0,
+                    c.independent_channels, c.stabilize_variance, c.average_channelsigma);
         return 0;
     }
     if (inpath.empty()) {
@@ -193,7 +198,8 @@ int main(int argc, char **argv)
     }
     if (ngpus > 1) std::fprintf(stderr, "inlm: %d GPUs requested; one GPU holds the volume, using device %d\n", ngpus, device);
 
-    const inlm_status st = inlm_denoise_files(&p, device, inpath.c_str(), outpath.c_str(), &o);
+    const inlm_status st = rgb ? inlm_denoise_rgb_files(&p, &c, device, inpath.c_str(), outpath.c_str(), &o)
+                               : inlm_denoise_files(&p, device, inpath.c_str(), outpath.c_str(), &o);
     if (st != INLM_OK) {
         std::fprintf(stderr, "inlm: %s (status %d)\n", inlm_last_error(nullptr), (int)st);
         return st == INLM_NO_DEVICE ? 3 : 1;

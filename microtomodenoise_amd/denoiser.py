@@ -292,3 +292,36 @@ def denoise_slabs(instack: np.ndarray, params: DenoiseParameters, slab_slices: i
     capi.check(st)
     return out, log[: params.maxiterations]
 
+
+def rgb_options(independent_channels=False, stabilize_variance=True, average_channelsigma=True):
+    """``inlm_rgb_options``: the reference's Color parameters
+    (denoise_parameters.h:63-67; --independent, --nopoisson, --noaverage)."""
+    c = capi.InlmRgbOptions()
+    capi.lib().inlm_default_rgb_options(C.byref(c))
+    c.independent_channels = int(independent_channels)
+    c.stabilize_variance = int(stabilize_variance)
+    c.average_channelsigma = int(average_channelsigma)
+    return c
+
+
+def denoise_rgb(image: np.ndarray, params: DenoiseParameters, device: int = 0, **color):
+    """One 2D RGB image ([3, ny, nx] channels, or [ny, nx, 3]) through the
+    colour path (Color/color_denoising.h:19-150; inlm_denoise_rgb): returns
+    ([3, ny, nx] float channels after the inverse Anscombe transform, the
+    three noise levels each iteration used)."""
+    img = np.asarray(image)
+    if img.ndim == 3 and img.shape[0] != 3 and img.shape[2] == 3:
+        img = np.moveaxis(img, 2, 0)
+    chans = np.ascontiguousarray(img, np.float32)
+    if chans.ndim != 3 or chans.shape[0] != 3:
+        raise ValueError("expected [3, ny, nx] or [ny, nx, 3] channels")
+    _, ny, nx = chans.shape
+    out = np.empty_like(chans)
+    log = np.zeros((max(params.maxiterations, 1), 3), np.float32)
+    sh = np.array([nx, ny], np.int32)
+    c = rgb_options(**color)
+    capi.check(capi.lib().inlm_denoise_rgb(C.byref(params.to_c()), C.byref(c), capi.ptr(sh, C.c_int32), device,
+                                           capi.ptr(chans, C.c_float), capi.ptr(out, C.c_float),
+                                           capi.ptr(log, C.c_float)))
+    return out, log[: params.maxiterations]
+

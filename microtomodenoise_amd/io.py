@@ -70,3 +70,36 @@ def denoise_files(inpath: str, params, outpath: Optional[str] = None, device: in
     o = io_options(**options)
     capi.check(capi.lib().inlm_denoise_files(C.byref(params.to_c()), device, _b(inpath),
                                              _b(outpath) if outpath else None, C.byref(o)))
+
+
+def read_rgb(path: str) -> np.ndarray:
+    """An RGB TIFF (8 or 16 bits, 3 or 4 samples per pixel) as [3, ny, nx]
+    float channels (hdcommunication.cpp:674-760)."""
+    sh = np.zeros(2, np.int32)
+    L = capi.lib()
+    capi.check(L.inlm_rgb_tiff_shape(_b(path), capi.ptr(sh, C.c_int32)))
+    out = np.empty((3, int(sh[1]), int(sh[0])), np.float32)
+    capi.check(L.inlm_rgb_tiff_read(_b(path), capi.ptr(out, C.c_float)))
+    return out
+
+
+def write_rgb(path: str, channels: np.ndarray):
+    """[3, ny, nx] channels to an 8-bit RGB TIFF, values capped at 255 and
+    truncated (color_denoising.h:72-77, Save2DTifImage_RGB)."""
+    ch = np.ascontiguousarray(channels, np.float32)
+    sh = np.array([ch.shape[2], ch.shape[1]], np.int32)
+    capi.check(capi.lib().inlm_rgb_tiff_write(_b(path), capi.ptr(ch, C.c_float), capi.ptr(sh, C.c_int32)))
+
+
+def denoise_rgb_files(inpath: str, params, outpath: Optional[str] = None, device: int = 0, independent_channels=False,
+                      stabilize_variance=True, average_channelsigma=True, **options):
+    """The reference program's colour mode (main.cpp:254-257) through
+    inlm_denoise_rgb_files: every RGB TIFF of inpath to
+    outpath/<name>_denoised.tif (and denoise_iterations/<name>_iterNN.tif
+    unless cleanup)."""
+    from .denoiser import rgb_options
+    o = io_options(**options)
+    c = rgb_options(independent_channels, stabilize_variance, average_channelsigma)
+    capi.check(capi.lib().inlm_denoise_rgb_files(C.byref(params.to_c()), C.byref(c), device, _b(inpath),
+                                                 _b(outpath) if outpath else None, C.byref(o)))
+

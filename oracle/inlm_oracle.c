@@ -314,6 +314,89 @@ ORC_API void orc_filter_f32(const float *Rpad, const float *Ppad, const int shap
     free(orig_all);
 }
 
+/* Three-channel filter of the colour path, Denoiser/iternlm_cpu.cpp:932-1075
+ * (filterslice_3channels), operation for operation.  Rpad/Ppad hold the three
+ * padded channels one after the other; mult[c] = -1/(sigma_c^2 beta)
+ * (iternlm_cpu.cpp:384-386).  Joint weights (independent == 0): the mean of
+ * the three scaled distances, (d0+d1+d2)/3, weights every channel; the
+ * centre enters once with the largest weight.  Independent channels: one
+ * weight per channel and no centre term (the reference's else-branch). */
+ORC_API void orc_filter_rgb_f32(const float *Rpad, const float *Ppad, const int shape[3], const int pad[3],
+                                const float mult[3], int independent, int nS, const long long *soff, int nP,
+                                const long long *poff, const float *w, float *out)
+{
+    const int nx = shape[0], ny = shape[1], nz = shape[2];
+    const long long nxp = nx + 2 * pad[0], nyp = ny + 2 * pad[1];
+    const long long nslp = nxp * nyp, nzp = nz + 2 * pad[2], npad = nslp * nzp;
+    const long long nvox = (long long)nx * ny * nz;
+    const float *R0 = Rpad, *R1 = Rpad + npad, *R2 = Rpad + 2 * npad;
+    const float *P0 = Ppad, *P1 = Ppad + npad, *P2 = Ppad + 2 * npad;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < ny; y++)
+        for (int z = 0; z < nz; z++)
+            for (int x = 0; x < nx; x++) {
+                const long long i0 = (z + pad[2]) * nslp + (y + pad[1]) * nxp + (x + pad[0]);
+                float fv0 = 0.f, fv1 = 0.f, fv2 = 0.f, fw0 = 0.f, fw1 = 0.f, fw2 = 0.f;
+                float wm0 = 0.f, wm1 = 0.f, wm2 = 0.f;
+                for (int k = 0; k < nS; k++) {
+                    const long long i1 = i0 + soff[k];
+                    float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+                    for (int p = 0; p < nP; p++) {
+                        const long long pp = poff[p];
+                        const float dw = w[p];
+                        float t = P0[i1 + pp] - P0[i0 + pp];
+                        d0 += (t * t) * dw;
+                        t = P1[i1 + pp] - P1[i0 + pp];
+                        d1 += (t * t) * dw;
+                        t = P2[i1 + pp] - P2[i0 + pp];
+                        d2 += (t * t) * dw;
+                    }
+                    d0 = d0 * mult[0];
+                    d1 = d1 * mult[1];
+                    d2 = d2 * mult[2];
+                    if (!independent) {
+                        const float ds = (d0 + d1 + d2) / 3.f;
+                        const float wt = pade_cut_f32(ds);
+                        fv0 += wt * R0[i1];
+                        fv1 += wt * R1[i1];
+                        fv2 += wt * R2[i1];
+                        fw0 += wt;
+                        if (wt > wm0) wm0 = wt;
+                    } else {
+                        const float w0 = pade_cut_f32(d0), w1 = pade_cut_f32(d1), w2 = pade_cut_f32(d2);
+                        fv0 += w0 * R0[i1];
+                        fv1 += w1 * R1[i1];
+                        fv2 += w2 * R2[i1];
+                        fw0 += w0;
+                        fw1 += w1;
+                        fw2 += w2;
+                        if (w0 > wm0) wm0 = w0;
+                        if (w1 > wm1) wm1 = w1;
+                        if (w2 > wm2) wm2 = w2;
+                    }
+                }
+                const long long o = ((long long)z * ny + y) * nx + x;
+                const float r0 = R0[i0], r1 = R1[i0], r2 = R2[i0];
+                if (!independent && wm0 > 0.f) {
+                    fv0 += wm0 * r0;
+                    fv1 += wm0 * r1;
+                    fv2 += wm0 * r2;
+                    fw0 += wm0;
+                    out[o] = fv0 / fw0;
+                    out[nvox + o] = fv1 / fw0;
+                    out[2 * nvox + o] = fv2 / fw0;
+                } else if (independent) {
+                    out[o] = wm0 > 0.f ? fv0 / fw0 : r0;
+                    out[nvox + o] = wm1 > 0.f ? fv1 / fw1 : r1;
+                    out[2 * nvox + o] = wm2 > 0.f ? fv2 / fw2 : r2;
+                } else {
+                    out[o] = r0;
+                    out[nvox + o] = r1;
+                    out[2 * nvox + o] = r2;
+                }
+            }
+}
+
 /* Weight mass per filter tile and search offset: the weights of
  * orc_filter_f32 (same FP32 operations, so the same values), summed in
  * double over the voxels of each tile of tile[0] x tile[1] x tile[2] voxels

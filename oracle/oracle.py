@@ -48,6 +48,9 @@ def orc():
         L.orc_pade_f32.restype = C.c_float
         L.orc_pade_f64.argtypes = [C.c_double]
         L.orc_pade_f64.restype = C.c_double
+        L.orc_filter_rgb_f32.argtypes = [_f32p, _f32p, _i32p, _i32p, _f32p, C.c_int, C.c_int, _i64p, C.c_int, _i64p,
+                                         _f32p, _f32p]
+        L.orc_filter_rgb_f32.restype = None
         L.orc_filter_f32.argtypes = [_f32p, _f32p, _i32p, _i32p, _f32p, C.c_float, C.c_int, _i64p, C.c_int,
                                      _i64p, _f32p, C.c_int, _f32p, C.c_int, C.c_int, C.c_int]
         L.orc_filter_f64.argtypes = [_f32p, _f32p, _i32p, _i32p, _f64p, C.c_double, C.c_int, _i64p, C.c_int,
@@ -226,6 +229,32 @@ def filter_iteration(R: np.ndarray, P: np.ndarray, sigma, p: Params, *, precisio
         orc().orc_filter_f64(Rp, Pp, shape, _i32(pad), sg, p.beta, len(soff), soff, len(poff), poff, w,
                              p.weight_mode, out, z0, z1, nthreads)
     return out
+
+
+def filter_rgb(R: np.ndarray, P: np.ndarray, channel_sigma, p: Params, independent=False) -> np.ndarray:
+    """One pass of the colour path's three-channel filter
+    (iternlm_cpu.cpp:932-1075), operation-faithful FP32.  R, P: [3, ny, nx]
+    (or [3, nz, ny, nx]) channels; channel_sigma: three noise levels.  The
+    colour path uses the computed patch weights (setup_distweight)."""
+    R = np.ascontiguousarray(R, np.float32)
+    P = np.ascontiguousarray(P, np.float32)
+    if R.ndim == 3:
+        R, P = R[:, None], P[:, None]
+    _, nz, ny, nx = R.shape
+    pad = p.padding()
+    Rp = np.ascontiguousarray(np.stack([pad_reflect(R[c], pad) for c in range(3)]))
+    Pp = np.ascontiguousarray(np.stack([pad_reflect(P[c], pad) for c in range(3)]))
+    _, nzp, nyp, nxp = Rp.shape
+    S = search_offsets(p.radius_searchspace, p.nslices)
+    PT = patch_offsets(p.radius_patchspace, p.nslices)
+    soff, poff = linear(S, nxp, nyp), linear(PT, nxp, nyp)
+    w = distweight(p.radius_patchspace, p.nslices, p.z_anisotropy, nzp)
+    s = np.asarray(channel_sigma, np.float32)
+    mult = np.array([np.float32(-1.0) / (s[c] * s[c] * np.float32(p.beta)) for c in range(3)], np.float32)
+    out = np.zeros(R.shape, np.float32)
+    orc().orc_filter_rgb_f32(Rp, Pp, _i32([nx, ny, nz]), _i32(pad), mult, int(independent), len(soff), soff,
+                             len(poff), poff, w, out)
+    return out[:, 0] if out.shape[1] == 1 else out
 
 
 def weight_mass(P: np.ndarray, sigma, p: Params, tile=(32, 8, 8)) -> np.ndarray:

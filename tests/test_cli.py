@@ -66,7 +66,10 @@ def test_defaults_are_the_reference_defaults():
     (("x", "--noblocks", "--blocks"), {"cpu_blockwise": "1"}),
     (("x", "-gpu0", "2"), {"device": "2"}),
     (("x", "-precision", "fp16", "-weight", "exp", "-seed", "9"), {"precision": "fp16", "weight": "exp", "seed": "9"}),
-    (("x", "--verbose", "-threads", "256", "-cpus", "8", "--unroll", "--independent"), {"verbose": "1"}),
+    (("x", "--verbose", "-threads", "256", "-cpus", "8", "--unroll"), {"verbose": "1"}),
+    (("x", "--rgb", "--independent", "--nopoisson", "--noaverage"),
+     {"rgb": "1", "independent_channels": "1", "stabilize_variance": "0", "average_channelsigma": "0"}),
+    (("x", "--color"), {"rgb": "1", "independent_channels": "0", "stabilize_variance": "1", "average_channelsigma": "1"}),
 ])
 def test_flags(args, want):
     d = params(*args)
@@ -93,8 +96,6 @@ def test_first_positional_argument_is_the_input():
 
 @pytest.mark.parametrize("args,msg", [
     (("x", "-search"), "missing value"),
-    (("x", "--rgb"), "RGB"),
-    (("x", "--color"), "RGB"),
     (("x", "-noise", "semimanual"), "noise mode"),
     (("x", "-mask", "900"), "mask"),
     (("x", "-precision", "fp8"), "precision"),
