@@ -39,6 +39,7 @@ struct inlm_handle {
     inlm::TapTable taps{};
     int rp_tiled = 0;
     size_t pad_elems = 0;  // allocated elements of each padded volume (>= padded voxels)
+    long shifted = 0;      // 16-bit forms: elements from a padded volume to its copy shifted by one
     // FP32 parity form: padded original and previous iterate in float.
     // Matrix-core forms: the same two volumes normalised, in the form's
     // storage type (Rs/Ps), plus the unpadded original in float.
@@ -223,6 +224,10 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         return fail(nullptr, INLM_BAD_ARG, "unknown weight function");
 
     inlm::Geometry g = inlm::make_geometry(shape, p.radius_searchspace, p.radius_patchspace, p.nslices);
+    // the 16-bit matrix-core forms keep an even row pitch (one unused column
+    // when nx + 2 px is odd), so that the parity of a load's element index
+    // depends on the search offset alone
+    if ((p.precision == INLM_PREC_FP16 || p.precision == INLM_PREC_BF16) && (g.nxp & 1) != 0) g.nxp += 1;
     const int padv[3] = {g.px, g.py, g.pz};
     for (int i = 0; i < 3; i++)
         if (shape[i] < 2 && padv[i] > 0)
@@ -304,9 +309,13 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         return bail(hip_fail(nullptr, e, "hipEventCreate"));
     h->pad_elems = padded_alloc_elems(g);
     // kGuard elements in front of each padded volume: the matrix-core staging
-    // reads 4-column chunks that start up to 3 elements before the volume.
+    // reads chunks that start up to 3 elements before the volume.  The
+    // 16-bit forms follow each volume with a copy shifted by one element
+    // (copy[j] = volume[j + 1]), 64-element aligned, whose own guard is the
+    // zeroed gap in front of it.
     const size_t esz = inlm::form_elem_size(h->p.precision);
-    const size_t padb = (h->pad_elems + kGuard) * esz;
+    if (h->fast && esz == 2) h->shifted = (long)((h->pad_elems + kGuard + 63) / 64 * 64);
+    const size_t padb = (h->shifted > 0 ? kGuard + (size_t)h->shifted + h->pad_elems : h->pad_elems + kGuard) * esz;
     if ((e = hipMalloc(&h->R_alloc, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(original)"));
     if ((e = hipMalloc(&h->P_alloc, padb)) != hipSuccess) return bail(hip_fail(nullptr, e, "hipMalloc(previous)"));
     if ((e = hipMemsetAsync(h->R_alloc, 0, padb, h->stream)) != hipSuccess ||
@@ -398,7 +407,7 @@ inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device)
         const double var = std::max(mom[1] / nvox - mean * mean, 0.0);
         h->shift = mean;
         h->scale = var > 0 ? 1.0 / std::sqrt(var) : 1.0;
-        HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->raw, h->Rs, h->fg, (float)h->shift,
+        HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->raw, h->Rs, h->shifted, h->fg, (float)h->shift,
                                               (float)h->scale, h->stream));
         HIP_TRY(h, hipStreamSynchronize(h->stream));
     }
@@ -430,7 +439,7 @@ inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, f
     if (iter > 1) {
         if (h->last_iter < 1) return fail(h, INLM_BAD_ARG, "iteration > 1 before iteration 1");
         if (h->fast)
-            HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->result, h->Ps, h->fg, (float)h->shift,
+            HIP_TRY(h, inlm::launch_pad_normalise(h->p.precision, h->result, h->Ps, h->shifted, h->fg, (float)h->shift,
                                                   (float)h->scale, h->stream));
         else
             HIP_TRY(h, inlm::launch_pad_reflect(h->result, h->P_pad, h->fg, h->stream));
@@ -449,7 +458,7 @@ inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, f
                                          h->prop_seed + 0x9E3779B97F4A7C15ull * (uint64_t)iter, h->lists_dev,
                                          h->counts_dev, h->stats_dev, h->stream));
         }
-        HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems,
+        HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems, h->shifted,
                                             h->raw, h->result, h->sigma, h->p.beta,
                                             (float)(1.0 / (h->scale * h->scale)), h->shift, 1.0 / h->scale,
                                             h->search_dev, ns, h->min_lin, h->taps, h->fg, h->stream,

@@ -74,7 +74,10 @@ hipError_t launch_patchstd(const float *origin, int64_t row_pitch, int64_t slice
 // (returned unchanged where every weight vanishes).
 bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n);
 // elems: allocated elements of each padded volume from Rs/Ps on (bounds the
-// buffer loads).
+// buffer loads).  shifted: for the 16-bit forms, the distance in elements
+// from each volume to its copy shifted by one element (copy[j] =
+// volume[j + 1], written by launch_pad_normalise); those forms need an even
+// row pitch g.nxp.  0 for FP32_FAST.
 // min_lin: the smallest linear offset of the search table (<= 0).
 // Per-tile offset lists for the matrix-core filter (32x8x8 tiles, x fastest):
 // tile t visits the search entries lists[t*K .. t*K + counts[t]) when
@@ -86,12 +89,12 @@ struct OffsetLists {
     long ntiles;
 };
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
-                              const float *raw, float *out, const float *sigma, float beta, float mscale,
+                              long shifted, const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
                               int min_lin, const TapTable &taps, const FilterGeom &g, hipStream_t st,
                               const OffsetLists *ol = nullptr);
-hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
-                                hipStream_t st);
+hipError_t launch_pad_normalise(int form, const float *in, void *out, long shifted, const FilterGeom &g, float shift,
+                                float scale, hipStream_t st);
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);
 inline size_t form_elem_size(int form) { return form == FORM_FP32 || form == FORM_FP32_FAST ? 4 : 2; }
 

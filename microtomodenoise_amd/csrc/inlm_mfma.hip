@@ -230,19 +230,43 @@ struct MfmaArgs {
     int K;
     int min_lin;  // smallest linear search offset (<= 0): the tile's buffer ranges start there
     float one;    // 1.0 (a runtime value the compiler cannot fold)
+    // 16-bit storage: R and P are each followed, `shifted` elements further
+    // on, by a copy shifted by one element (copy[j] = volume[j + 1])
+    long shifted;
 };
 
-template <int R>
+// Staging geometry.  A region row (slice Z0-R+rz, row Y0-R+ry) is staged
+// as 10 units of 4 columns, LDS columns [4, 44).  Every load is one 16-byte
+// (or, for the last 16-bit units, 8-byte) buffer load at a 4-byte-aligned
+// address: on gfx950 such a load costs the texture address unit the same
+// cycles whether it carries 8 or 16 bytes, while a 2-byte-aligned one costs
+// two to four times as many.  16-bit storage therefore stages chunks of 8
+// columns (two units) in S8 slots of NTM chunks and any remainder as 4-column
+// chunks in S4 slots; FP32 storage stages 4-column chunks (16 bytes).
+template <int R, class TS>
 struct MTile {
     static constexpr int RY = MTY + 2 * R, RZ = MTZ + 2 * R;
-    static constexpr int CPR = 10;  // 4-column chunks staged per row: columns [4, 44)
-    static constexpr int NCHUNK = RZ * RY * CPR;
-    static constexpr int CPT = (NCHUNK + NTM - 1) / NTM;
-    // staging slots past NCHUNK write to their own 4 elements behind the
-    // plane, so every thread stages CPT chunks with no per-chunk branch
+    static constexpr int UPR = 10;  // 4-column units per region row
+    static constexpr int NUNIT = RZ * RY * UPR;
+    static constexpr bool WIDE = sizeof(TS) == 2;
+    static constexpr int S8A = WIDE ? NUNIT / (2 * NTM) : 0;
+    static constexpr int REM = NUNIT - 2 * NTM * S8A;
+    // a remainder larger than one slot of 4-column chunks takes another slot
+    // of 8-column chunks
+    static constexpr bool REM8 = WIDE && REM > NTM;
+    static constexpr int S8 = S8A + (REM8 ? 1 : 0);
+    static constexpr int N8 = REM8 ? NUNIT / 2 : S8A * NTM;
+    static constexpr int N4 = NUNIT - 2 * N8;
+    static constexpr int S4 = (N4 + NTM - 1) / NTM;
+    static constexpr int CPT = S8 + S4;
+    // staging slots past N8 (N4) write to their own 8 (4) elements behind the
+    // plane, so every thread stages its chunks with no per-chunk branch
     static constexpr int PLANE = RZ * RYP * XW;
-    static constexpr int LDS = PLANE + 4 * (CPT * NTM - NCHUNK);  // elements per buffer
+    static constexpr int SPARE8 = PLANE;
+    static constexpr int SPARE4 = PLANE + 8 * (S8 * NTM - N8);
+    static constexpr int LDS = SPARE4 + 4 * (S4 * NTM - N4);  // elements per buffer
     static_assert(RY <= RYP, "patch radius too large for the padded slice");
+    static_assert(UPR % 2 == 0, "8-column chunks must not straddle region rows");
 };
 
 __device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
@@ -263,7 +287,7 @@ __device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
 // vector instructions per output and offset in the weight step (distance
 // scale, polynomial, cutoff test and select, the three sums)
 constexpr int kWeightValu = 13;
-// per staged chunk of four values (difference, square, clamp; packed for FP16)
+// per staged unit of four values (difference, square, clamp; packed for FP16)
 template <class TS>
 constexpr int kStageValu = sizeof(TS) == 2 ? 6 : 12;
 // per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
@@ -355,68 +379,77 @@ __device__ __forceinline__ void load4(const TS *p, float (&v)[4])
     }
 }
 
-// Staging of D = (P(y+s) - P(y))^2 into the 16-bit D plane, 4 columns per
-// chunk.  Generic form: FP32 arithmetic, clamped below the FP16 maximum when
-// the plane is FP16 (an infinite D would turn the zero band weights of its
-// neighbours into NaN).
-template <int R, class TS, class TD>
+// Staging of D = (P(y+s) - P(y))^2 into the 16-bit D plane, W columns per
+// chunk (W * sizeof(TS) is 8 or 16 bytes: one buffer load).  Generic form:
+// FP32 arithmetic, clamped below the FP16 maximum when the plane is FP16 (an
+// infinite D would turn the zero band weights of its neighbours into NaN).
+template <int W, class TS>
+__device__ __forceinline__ void load_chunk(void *c, rsrc_t r, unsigned off, unsigned soff)
+{
+    // the builtins return GCC-style vectors; copy them out whole (element
+    // subscripts on them are not lowered correctly by this compiler)
+    static_assert(W * sizeof(TS) == 8 || W * sizeof(TS) == 16, "one 8- or 16-byte load per chunk");
+    if constexpr (W * sizeof(TS) == 16) {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0);
+        __builtin_memcpy(c, &q, 16);
+    } else {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+        __builtin_memcpy(c, &q, 8);
+    }
+}
+// 8-byte-aligned LDS stores of a chunk's W 16-bit D values
+template <int W>
+__device__ __forceinline__ void store_dplane(void *dst, const void *src)
+{
+#pragma unroll
+    for (int h = 0; h < W / 4; h++) __builtin_memcpy(static_cast<char *>(dst) + 8 * h, static_cast<const char *>(src) + 8 * h, 8);
+}
+
+template <int W, class TS, class TD>
 struct Stager {
-    static constexpr int CPT = MTile<R>::CPT;
-    typedef TS chunk[4];  // stored values, widened to FP32 when staged
-    const chunk *p0;      // this thread's chunks at the tile's own positions (LDS)
-    __device__ __forceinline__ void load(chunk &c, const TS *p) const { __builtin_memcpy(c, p, 4 * sizeof(TS)); }
+    typedef TS chunk[W];  // stored values, widened to FP32 when staged
+    __device__ __forceinline__ void load(chunk &c, const TS *p) const { __builtin_memcpy(c, p, sizeof(chunk)); }
     __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
-        // the builtins return GCC-style vectors; copy them out whole (element
-        // subscripts on them are not lowered correctly by this compiler)
-        if constexpr (sizeof(TS) == 4) {
-            const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0);
-            __builtin_memcpy(c, &q, 16);
-        } else {
-            const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
-            __builtin_memcpy(c, &q, 8);
-        }
+        load_chunk<W, TS>(&c, r, off, soff);
     }
-    __device__ __forceinline__ void store(const chunk &pn, int c, TD *dst) const
+    // pn: the chunk at the shifted position; b: the chunk at the tile's own
+    // position (from LDS)
+    __device__ __forceinline__ void store(const chunk &pn, const chunk &b, TD *dst) const
     {
-        TD dv[4];
-        chunk b;
-        __builtin_memcpy(b, &p0[c * NTM], sizeof(chunk));
+        TD dv[W];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < W; e++) {
             const float df = (float)pn[e] - (float)b[e];
             float dd = df * df;
             if constexpr (std::is_same<TD, _Float16>::value) dd = __builtin_fminf(dd, 65504.f);
             dv[e] = DPlane<TD>::from(dd);
         }
-        __builtin_memcpy(dst, dv, 8);
+        store_dplane<W>(dst, dv);
     }
 };
 
 // FP16 storage and FP16 plane: packed half arithmetic, two values per
 // instruction (difference, square, clamp).
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-template <int R>
-struct Stager<R, _Float16, _Float16> {
-    static constexpr int CPT = MTile<R>::CPT;
-    typedef half2v chunk[2];
-    const chunk *p0;  // this thread's chunks at the tile's own positions (LDS)
-    __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, 8); }
+template <int W>
+struct Stager<W, _Float16, _Float16> {
+    typedef half2v chunk[W / 2];
+    __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, sizeof(chunk)); }
     __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
-        const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
-        __builtin_memcpy(&c[0], &q, 8);
+        load_chunk<W, _Float16>(&c[0], r, off, soff);
     }
-    __device__ __forceinline__ void store(const chunk &pn, int c, _Float16 *dst) const
+    __device__ __forceinline__ void store(const chunk &pn, const chunk &b, _Float16 *dst) const
     {
         const half2v big = {(_Float16)65504.f, (_Float16)65504.f};
-        chunk b;
-        __builtin_memcpy(b, &p0[c * NTM], sizeof(chunk));
-        half2v d0 = pn[0] - b[0], d1 = pn[1] - b[1];
-        d0 = __builtin_elementwise_min(d0 * d0, big);
-        d1 = __builtin_elementwise_min(d1 * d1, big);
-        half2v out[2] = {d0, d1};
-        __builtin_memcpy(dst, out, 8);
+        half2v out[W / 2];
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) {
+            const half2v d = pn[h] - b[h];
+            out[h] = __builtin_elementwise_min(d * d, big);
+        }
+        store_dplane<W>(dst, out);
     }
 };
 
@@ -430,7 +463,7 @@ template <int R, class TS, class TD, int WFN, bool PRUNED, bool ISO>
 __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R, TS>(), mfma_waves_per_eu<R, TS>())))
 void filter_mfma(MfmaArgs<TS> a)
 {
-    using T = MTile<R>;
+    using T = MTile<R, TS>;
     using DP = DPlane<TD>;
     typedef typename DP::vec8 vec8;
     constexpr const MRows<R> &rows = MRowsOf<R>::value;
@@ -473,27 +506,55 @@ void filter_mfma(MfmaArgs<TS> a)
         }
     }
 
-    // Staging chunks: 4 columns of one region row.  Region row (rz, ry) is
-    // slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
+    // Staging chunks: 8 or 4 columns of one region row.  Region row (rz, ry)
+    // is slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
     const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
-    using ST = Stager<R, TS, TD>;
+    constexpr int W8 = sizeof(TS) == 2 ? 8 : 4;
+    using ST8 = Stager<W8, TS, TD>;
+    using ST4 = Stager<4, TS, TD>;
+    typedef typename ST8::chunk chunk8;
+    typedef typename ST4::chunk chunk4;
+    constexpr int S8 = T::S8, S4 = T::S4;
     // the chunks at the tile's own positions stay fixed over the offsets and
     // live in LDS, one slot per (thread, chunk)
-    __shared__ __attribute__((aligned(16))) typename ST::chunk p0s[T::CPT * NTM];
-    ST stg;
-    stg.p0 = &p0s[t];
-    int coff[T::CPT], loff[T::CPT];
-#pragma unroll
-    for (int c = 0; c < T::CPT; c++) {
-        const int cs = t + c * NTM;
-        const int ci = cs < T::NCHUNK ? cs : 0;  // spare slots re-read chunk 0
-        const int cx = ci % T::CPR, row = ci / T::CPR;
+    __shared__ __attribute__((aligned(16))) chunk8 p0s8[S8 > 0 ? S8 * NTM : 1];
+    __shared__ __attribute__((aligned(16))) chunk4 p0s4[S4 > 0 ? S4 * NTM : 1];
+    ST8 stg8;
+    ST4 stg4;
+    int coff8[S8 > 0 ? S8 : 1], loff8[S8 > 0 ? S8 : 1], coff4[S4 > 0 ? S4 : 1], loff4[S4 > 0 ? S4 : 1];
+    // unit u (4 columns) of the region: its volume offset from Pt and its
+    // LDS element offset
+    auto place = [&](int u, int &co, int &lo) __attribute__((always_inline)) {
+        const int row = u / T::UPR, col = 4 * (u % T::UPR);
         const int ry = row % T::RY, rz = row / T::RY;
-        coff[c] = (int)(rz * sp + (long)ry * g.nxp + 4 * cx);
-        loff[c] = cs < T::NCHUNK ? (rz * RYP + ry) * XW + 4 + 4 * cx : T::PLANE + 4 * (cs - T::NCHUNK);
-        typename ST::chunk b;
-        stg.load(b, Pt + coff[c]);
-        __builtin_memcpy(&p0s[t + c * NTM], b, sizeof(b));
+        co = (int)(rz * sp + (long)ry * g.nxp + col);
+        lo = (rz * RYP + ry) * XW + 4 + col;
+    };
+#pragma unroll
+    for (int c = 0; c < S8; c++) {
+        const int cs = t + c * NTM;
+        if (cs < T::N8) {
+            place(2 * cs, coff8[c], loff8[c]);
+        } else {  // spare slots re-read chunk 0
+            coff8[c] = 0;
+            loff8[c] = T::SPARE8 + 8 * (cs - T::N8);
+        }
+        chunk8 b;
+        stg8.load(b, Pt + coff8[c]);
+        __builtin_memcpy(&p0s8[cs], b, sizeof(b));
+    }
+#pragma unroll
+    for (int c = 0; c < S4; c++) {
+        const int cs = t + c * NTM;
+        if (cs < T::N4) {
+            place(2 * T::N8 + cs, coff4[c], loff4[c]);
+        } else {
+            coff4[c] = 0;
+            loff4[c] = T::SPARE4 + 4 * (cs - T::N4);
+        }
+        chunk4 b;
+        stg4.load(b, Pt + coff4[c]);
+        __builtin_memcpy(&p0s4[cs], b, sizeof(b));
     }
 
     // This lane's outputs.  The MFMA forms the transposed block,
@@ -531,33 +592,61 @@ void filter_mfma(MfmaArgs<TS> a)
     const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f));
     const half2w msu = {msu1, msu1};
 
-    // One buffer resource per volume and tile, starting at the smallest
-    // search offset: a search offset is then the uniform (scalar) part of the
+    // Buffer resources per volume and tile, starting at the smallest search
+    // offset: a search offset is then the uniform (scalar) part of the
     // address and the per-lane part stays fixed.  Every address lies inside
-    // the allocation; reads past its end would return zero.
+    // the allocation; reads past its end would return zero.  With 16-bit
+    // storage the row and slice pitches are even, so every load of one
+    // offset has the parity of px + lin in its element index; an odd one
+    // reads the shifted copy one element earlier, so that every load address
+    // is 4-byte aligned.
     const TS *Pb = Pt + a.min_lin;
     const TS *Rb = Rt + a.min_lin;
-    const rsrc_t rP = make_rsrc(Pb, (long)(a.P_end - Pb) * (long)sizeof(TS));
-    const rsrc_t rR = make_rsrc(Rb, (long)(a.R_end - Rb) * (long)sizeof(TS));
+    const TS *Pb1 = Pb + (a.shifted - 1);
+    const TS *Rb1 = Rb + (a.shifted - 1);
+    auto records = [](long bytes) __attribute__((always_inline)) {
+        const unsigned long b = (unsigned long)bytes;
+        return (b >> 31) != 0 ? 0x7fffffff : (int)b;
+    };
+    const int nP = records((long)(a.P_end - Pb) * (long)sizeof(TS));
+    const int nR = records((long)(a.R_end - Rb) * (long)sizeof(TS));
+    auto odd_of = [&](int lin) __attribute__((always_inline)) {
+        if constexpr (sizeof(TS) == 2) return ((lin + g.px) & 1) != 0;
+        return false;
+    };
+    auto rsrc_p = [&](int lin) __attribute__((always_inline)) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<TS *>(odd_of(lin) ? Pb1 : Pb), (short)0, nP, kRsrcWord3);
+    };
+    auto rsrc_r = [&](int lin) __attribute__((always_inline)) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<TS *>(odd_of(lin) ? Rb1 : Rb), (short)0, nR, kRsrcWord3);
+    };
     auto soff_of = [&](int lin) __attribute__((always_inline)) {
         return (unsigned)((lin - a.min_lin) * (int)sizeof(TS));
     };
-    typedef typename ST::chunk chunk;
-    auto load_chunks = [&](chunk (&pn)[T::CPT], int lin) __attribute__((always_inline)) {
+    chunk8 pn8[S8 > 0 ? S8 : 1];
+    chunk4 pn4[S4 > 0 ? S4 : 1];
+    auto load_chunks = [&](int lin) __attribute__((always_inline)) {
+        const rsrc_t r = rsrc_p(lin);
         const unsigned so = soff_of(lin);
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++) stg.load(pn[c], rP, (unsigned)(coff[c] * (int)sizeof(TS)), so);
-    };
-    auto stage = [&](const chunk (&pn)[T::CPT], TD *buf) __attribute__((always_inline)) {
+        for (int c = 0; c < S8; c++) stg8.load(pn8[c], r, (unsigned)(coff8[c] * (int)sizeof(TS)), so);
 #pragma unroll
-        for (int c = 0; c < T::CPT; c++) stg.store(pn[c], c, buf + loff[c]);
+        for (int c = 0; c < S4; c++) stg4.load(pn4[c], r, (unsigned)(coff4[c] * (int)sizeof(TS)), so);
+    };
+    auto stage = [&](TD *buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < S8; c++) stg8.store(pn8[c], p0s8[t + c * NTM], buf + loff8[c]);
+#pragma unroll
+        for (int c = 0; c < S4; c++) stg4.store(pn4[c], p0s4[t + c * NTM], buf + loff4[c]);
     };
     // R at this lane's four outputs shifted by one search offset, in the
     // stored type
     struct RVals {
         TS v[4];
     };
-    auto load_r = [&](RVals &rv, unsigned so) __attribute__((always_inline)) {
+    auto load_r = [&](RVals &rv, int lin) __attribute__((always_inline)) {
+        const rsrc_t rR = rsrc_r(lin);
+        const unsigned so = soff_of(lin);
         if constexpr (sizeof(TS) == 4) {
             const auto v = __builtin_amdgcn_raw_buffer_load_b128(rR, rbyte, so, 0);
             __builtin_memcpy(rv.v, &v, 16);
@@ -624,12 +713,11 @@ void filter_mfma(MfmaArgs<TS> a)
     // for the last offset).  Steps come in pairs that swap the two distance
     // registers and the two LDS buffers; when nS is odd the last step
     // repeats offset nS-1 and its distances are dropped.
-    chunk pn[T::CPT];
     __syncthreads();  // zeroed LDS visible before the first staging
     auto lin_at = [&](int k) __attribute__((always_inline)) { return lin_of(k < nS ? k : nS - 1); };
-    load_chunks(pn, lin_at(0));
-    stage(pn, lds[0]);
-    load_chunks(pn, lin_at(1));
+    load_chunks(lin_at(0));
+    stage(lds[0]);
+    load_chunks(lin_at(1));
     __syncthreads();
 
     // no offset before the first: an infinite distance has weight zero
@@ -647,7 +735,7 @@ void filter_mfma(MfmaArgs<TS> a)
                     const RVals &rin) __attribute__((always_inline)) {
         const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
         const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
-        load_r(rout, soff_of(l0));
+        load_r(rout, l0);
         const TD *L = rd + aoff;
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         // A operands in flight: KD reads ahead of the MFMA that consumes them,
@@ -672,11 +760,11 @@ void filter_mfma(MfmaArgs<TS> a)
             if constexpr (r + KD < NR) ab[r % KD] = a_of(std::integral_constant<int, r + KD>{});
         });
         accumulate(din, rin);
-        stage(pn, st);
+        stage(st);
         // order: the R load, KD reads, then per MFMA the next A read and a
         // share of the vector work (none right after the first MFMA); the
         // staging writes and the next loads follow the chain
-        constexpr int NV = ((kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu) + kStageValu<TS> * T::CPT + NR - 2) /
+        constexpr int NV = ((kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu) + kStageValu<TS> * (2 * T::S8 + T::S4) + NR - 2) /
                            (NR - 1);
         __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
         __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
@@ -687,7 +775,7 @@ void filter_mfma(MfmaArgs<TS> a)
         });
         // the loads below overwrite the chunks the staging has just read
         __builtin_amdgcn_sched_barrier(0);
-        load_chunks(pn, l2);
+        load_chunks(l2);
         __syncthreads();
         dout = acc;
     };
@@ -734,7 +822,7 @@ void filter_mfma(MfmaArgs<TS> a)
 // ---------------------------------------------------------------------------
 template <class TS>
 __global__ __launch_bounds__(256) void pad_normalise_kernel(const float *__restrict__ in, TS *__restrict__ out,
-                                                            FilterGeom g, float shift, float scale)
+                                                            long shifted, FilterGeom g, float shift, float scale)
 {
     const long nn = (long)g.nxp * g.nyp * g.nzp;
     const long idx = (long)blockIdx.x * 256 + threadIdx.x;
@@ -747,7 +835,9 @@ __global__ __launch_bounds__(256) void pad_normalise_kernel(const float *__restr
     while (yy < 0 || yy >= g.ny) yy = (yy < 0) ? -yy : 2 * g.ny - yy - 2;
     while (zz < 0 || zz >= g.nz) zz = (zz < 0) ? -zz : 2 * g.nz - zz - 2;
     const float v = in[((long)zz * g.ny + yy) * g.nx + xx];
-    out[idx] = (TS)((v - shift) * scale);
+    const TS o = (TS)((v - shift) * scale);
+    out[idx] = o;
+    if (shifted > 0) out[shifted + idx - 1] = o;  // the copy shifted by one element
 }
 
 // Sum and sum of squares of a float volume in FP64 (one atomic pair per
@@ -807,12 +897,20 @@ bool mfma_bands_r(const int *dxyz, const float *w, int ntap, float (*band)[8])
 }
 
 template <int R, class TS, class TD, int WFN>
-hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
+hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
                          const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
                          const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
+    if constexpr (sizeof(TS) == 2) {
+        // 16-bit loads stay 4-byte aligned only with an even row pitch and
+        // the shifted copies behind the volumes
+        if ((g.nxp & 1) != 0 || shifted < (long)elems || (shifted & 1) != 0) return hipErrorInvalidValue;
+    } else if (shifted != 0) {
+        return hipErrorInvalidValue;
+    }
     MfmaArgs<TS> a;
+    a.shifted = shifted;
     a.min_lin = min_lin;
     a.one = 1.f;
     a.lists = ol ? ol->lists : nullptr;
@@ -853,20 +951,21 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, const flo
 }
 
 template <int R, int WFN>
-hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, const float *raw, float *out,
+hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw,
+                            float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
                             const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
                             const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
-        return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+        return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                       inv_scale, search, n_search, min_lin, taps, g, ol, st);
     case FORM_FP16:
-        return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+        return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                          inv_scale, search, n_search, min_lin, taps, g, ol, st);
     case FORM_BF16:
-        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, raw, out, sigma, beta, mscale, shift,
+        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                      inv_scale, search, n_search, min_lin, taps, g, ol, st);
     default: return hipErrorInvalidValue;
     }
@@ -886,16 +985,16 @@ bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
 }
 
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
-                              const float *raw, float *out, const float *sigma, float beta, float mscale,
+                              long shifted, const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search, int n_search,
                               int min_lin, const TapTable &taps, const FilterGeom &g, hipStream_t st,
                               const OffsetLists *ol)
 {
     if (min_lin > 0) return hipErrorInvalidValue;
 #define INLM_MF(RR)                                                                                               \
-    return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
+    return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
                                                     search, n_search, min_lin, taps, g, ol, st)                                 \
-                          : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, raw, out, sigma, beta, mscale, shift, inv_scale, \
+                          : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
                                                     search, n_search, min_lin, taps, g, ol, st)
     switch (rp) {
     case 1: INLM_MF(1);
@@ -906,23 +1005,24 @@ hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, c
 #undef INLM_MF
 }
 
-hipError_t launch_pad_normalise(int form, const float *in, void *out, const FilterGeom &g, float shift, float scale,
-                                hipStream_t st)
+hipError_t launch_pad_normalise(int form, const float *in, void *out, long shifted, const FilterGeom &g, float shift,
+                                float scale, hipStream_t st)
 {
     const long nn = (long)g.nxp * g.nyp * g.nzp;
     const unsigned blocks = (unsigned)((nn + 255) / 256);
     switch (form) {
     case FORM_FP32_FAST:
-        hipLaunchKernelGGL(pad_normalise_kernel<float>, dim3(blocks), dim3(256), 0, st, in, (float *)out, g, shift,
-                           scale);
-        break;
-    case FORM_FP16:
-        hipLaunchKernelGGL(pad_normalise_kernel<_Float16>, dim3(blocks), dim3(256), 0, st, in, (_Float16 *)out, g,
+        if (shifted != 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(pad_normalise_kernel<float>, dim3(blocks), dim3(256), 0, st, in, (float *)out, 0L, g,
                            shift, scale);
         break;
+    case FORM_FP16:
+        hipLaunchKernelGGL(pad_normalise_kernel<_Float16>, dim3(blocks), dim3(256), 0, st, in, (_Float16 *)out,
+                           shifted, g, shift, scale);
+        break;
     case FORM_BF16:
-        hipLaunchKernelGGL(pad_normalise_kernel<__bf16>, dim3(blocks), dim3(256), 0, st, in, (__bf16 *)out, g, shift,
-                           scale);
+        hipLaunchKernelGGL(pad_normalise_kernel<__bf16>, dim3(blocks), dim3(256), 0, st, in, (__bf16 *)out, shifted,
+                           g, shift, scale);
         break;
     default: return hipErrorInvalidValue;
     }
