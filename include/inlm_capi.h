@@ -54,8 +54,17 @@ typedef enum { INLM_NOISE_GLOBAL = 0, INLM_NOISE_ZADAPTIVE = 1, INLM_NOISE_MANUA
  *                Pade; differs from FP32 by rounding only.
  *   BF16/FP16 -- R and P stored in 16 bits after the affine map
  *                (v - shift) * scale; differences, sums and accumulators in
- *                FP32, normalisation in FP64. */
-typedef enum { INLM_PREC_FP32 = 0, INLM_PREC_FP32_FAST = 1, INLM_PREC_BF16 = 2, INLM_PREC_FP16 = 3 } inlm_precision;
+ *                FP32, normalisation in FP64.
+ *   FP16_FP8  -- FP16 storage as FP16, squared-difference planes in FP8
+ *                e4m3 scaled per tile by a power of two, patch weights in
+ *                FP8 times a common factor; distance sums in FP32. */
+typedef enum {
+    INLM_PREC_FP32 = 0,
+    INLM_PREC_FP32_FAST = 1,
+    INLM_PREC_BF16 = 2,
+    INLM_PREC_FP16 = 3,
+    INLM_PREC_FP16_FP8 = 4
+} inlm_precision;
 
 /* Weight function: the reference CPU's Pade [3/3] with the -3.56648 cutoff
  * (Denoiser/iternlm_cpu.h:125-126) or the reference GPU's expf

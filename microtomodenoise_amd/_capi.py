@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libinlm.so")
 
 INLM_OK, INLM_NO_DEVICE, INLM_HIP_ERROR, INLM_BAD_ARG, INLM_OOM, INLM_IO_ERROR = 0, -1, -2, -3, -4, -5
 NOISE_GLOBAL, NOISE_ZADAPTIVE, NOISE_MANUAL = 0, 1, 2
-PREC_FP32, PREC_FP32_FAST, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
+PREC_FP32, PREC_FP32_FAST, PREC_BF16, PREC_FP16, PREC_FP16_FP8 = 0, 1, 2, 3, 4
 WEIGHT_PADE, WEIGHT_EXP = 0, 1
 
 STATUS_NAMES = {0: "OK", -1: "NO_DEVICE", -2: "HIP_ERROR", -3: "BAD_ARG", -4: "OOM", -5: "IO_ERROR"}

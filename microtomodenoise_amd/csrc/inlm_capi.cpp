@@ -218,7 +218,7 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         p.radius_searchspace[2] = 0;
         p.radius_patchspace[2] = 0;
     }
-    if (p.precision < INLM_PREC_FP32 || p.precision > INLM_PREC_FP16)
+    if (p.precision < INLM_PREC_FP32 || p.precision > INLM_PREC_FP16_FP8)
         return fail(nullptr, INLM_BAD_ARG, "unknown precision form");
     if (p.weight_fn != INLM_WEIGHT_PADE && p.weight_fn != INLM_WEIGHT_EXP)
         return fail(nullptr, INLM_BAD_ARG, "unknown weight function");
@@ -227,7 +227,7 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
     // the 16-bit matrix-core forms keep an even row pitch (one unused column
     // when nx + 2 px is odd), so that the parity of a load's element index
     // depends on the search offset alone
-    if ((p.precision == INLM_PREC_FP16 || p.precision == INLM_PREC_BF16) && (g.nxp & 1) != 0) g.nxp += 1;
+    if (inlm::form_elem_size(p.precision) == 2 && (g.nxp & 1) != 0) g.nxp += 1;
     const int padv[3] = {g.px, g.py, g.pz};
     for (int i = 0; i < 3; i++)
         if (shape[i] < 2 && padv[i] > 0)

@@ -19,7 +19,7 @@ struct FilterGeom {
     int px, py, pz;
 };
 
-enum FilterForm { FORM_FP32 = 0, FORM_FP32_FAST = 1, FORM_BF16 = 2, FORM_FP16 = 3 };
+enum FilterForm { FORM_FP32 = 0, FORM_FP32_FAST = 1, FORM_BF16 = 2, FORM_FP16 = 3, FORM_FP16_FP8 = 4 };
 
 constexpr int kMaxTaps = 128;
 constexpr int kMfmaMaxPatterns = 16;

@@ -29,7 +29,7 @@ const char *kUsage =
     "  noise:    -noise/-noisemode global|z-adaptive|manual  -sigma/-s F  -sigma0/-s0 F  -sigma1/-s1 F\n"
     "            -nsamples/-noise_samples N  -noisepatch N  -noiseshift F  --continuous  --preserve_detail\n"
     "  hardware: -gpu0/-device N  -gpus N (one GPU is used)  --blocks  --noblocks  --verbose/--v\n"
-    "  this build: -precision fp32|fp32_fast|fp16|bf16  -weight pade|exp  -seed N  --print-params\n";
+    "  this build: -precision fp32|fp32_fast|fp16|bf16|fp16_fp8  -weight pade|exp  -seed N  --print-params\n";
 
 bool is(const char *a, const char *b) { return std::strcmp(a, b) == 0; }
 
@@ -45,7 +45,11 @@ const char *noise_name(int m)
 }
 const char *precision_name(int p)
 {
-    return p == INLM_PREC_FP32_FAST ? "fp32_fast" : p == INLM_PREC_FP16 ? "fp16" : p == INLM_PREC_BF16 ? "bf16" : "fp32";
+    return p == INLM_PREC_FP32_FAST ? "fp32_fast"
+           : p == INLM_PREC_FP16    ? "fp16"
+           : p == INLM_PREC_BF16    ? "bf16"
+           : p == INLM_PREC_FP16_FP8 ? "fp16_fp8"
+                                     : "fp32";
 }
 
 }  // namespace
@@ -153,6 +157,7 @@ int main(int argc, char **argv)
             if (is(v, "fp32")) p.precision = INLM_PREC_FP32;
             else if (is(v, "fp32_fast")) p.precision = INLM_PREC_FP32_FAST;
             else if (is(v, "fp16")) p.precision = INLM_PREC_FP16;
+            else if (is(v, "fp16_fp8")) p.precision = INLM_PREC_FP16_FP8;
             else if (is(v, "bf16")) p.precision = INLM_PREC_BF16;
             else return fail("unknown precision", v);
         }

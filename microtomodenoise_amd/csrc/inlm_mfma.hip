@@ -20,6 +20,9 @@
 // offset); wavefront w computes the 16x16 block x-half (w & 1), z pair (w >> 1).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "inlm_kernels.hpp"
@@ -38,9 +41,18 @@ constexpr int MTX = 32, MTY = 8, MTZ = 8;
 // pitch with 12 rows per slice took three times as many LDS cycles per read.
 constexpr int XW = 48;
 constexpr int RYP = 16;
+// FP8 planes (1-byte elements) use 48-byte rows and 24 rows per slice: rows
+// m and m + 8 of an operand then fall 8 bank groups apart, and a slot's 16
+// rows hit 16 distinct 16-byte bank groups.
+constexpr int XW8 = 48;
+constexpr int RYP8 = 24;
 constexpr float kCut = -3.56648f;
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+// OCP FP8 e4m3 D-plane element
+typedef unsigned char f8_t;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -174,6 +186,91 @@ struct MRowsOf {
     static_assert(value.nops <= kMaxRowOps && value.npd <= 4, "too many MFMA operations");
 };
 
+// FP8 planes: one v_mfma_f32_16x16x128_f8f6f4 sums four 32-byte K slots,
+// slot s taken by lane group s (lanes 16s .. 16s+15) of both operands.  A
+// slot holds one item of the 16-bit plan: a patch row, read over input
+// columns x0-4 .. x0+27, or a pair of one-tap rows, x0 .. x0+15 of each.
+// Items are ordered by how often their band key occurs and laid out two
+// slots at a time over pairs of MFMAs, so MFMAs whose slots carry the same
+// keys share one band operand (at r = 2: three MFMAs, two band operands).
+template <int R, bool ISO>
+struct F8Plan {
+    static constexpr int kMaxItems = 4 * R * R + 4 * R + 1;
+    static constexpr int kMaxM = (kMaxItems + 3) / 4;
+    int ni;
+    int ia[kMaxItems], ib[kMaxItems], key[kMaxItems];
+    int nm;
+    int slot[kMaxM][4];  // item of each MFMA slot, -1 for an empty slot
+    bool haspair[kMaxM];
+    int nsig;
+    int sig[kMaxM];   // band operand of each MFMA
+    int sigm[kMaxM];  // the MFMA whose slots define each band operand
+    constexpr F8Plan() : ni(0), ia(), ib(), key(), nm(0), slot(), haspair(), nsig(0), sig(), sigm()
+    {
+        constexpr const MRows<R> &rows = MRowsOf<R>::value;
+        for (int o = 0; o < rows.nops; o++) {
+            const int p = rows.r[rows.opa[o]].pat;
+            ia[ni] = rows.opa[o];
+            ib[ni] = rows.opb[o];
+            key[ni] = (rows.opb[o] >= 0 ? 1000 : 0) + (ISO ? rows.cls[p] : p);
+            ni++;
+        }
+        int order[kMaxItems] = {}, cnt[kMaxItems] = {};
+        for (int i = 0; i < ni; i++) {
+            order[i] = i;
+            for (int k = 0; k < ni; k++) cnt[i] += key[k] == key[i] ? 1 : 0;
+        }
+        // insertion sort by (descending count, key, item)
+        for (int i = 1; i < ni; i++) {
+            const int v = order[i];
+            int j = i - 1;
+            while (j >= 0) {
+                const int u = order[j];
+                const bool before = cnt[v] != cnt[u] ? cnt[v] > cnt[u] : (key[v] != key[u] ? key[v] < key[u] : v < u);
+                if (!before) break;
+                order[j + 1] = u;
+                j--;
+            }
+            order[j + 1] = v;
+        }
+        nm = (ni + 3) / 4;
+        for (int o = 0; o < kMaxM; o++)
+            for (int q = 0; q < 4; q++) slot[o][q] = -1;
+        int pos = 0;
+        for (int b = 0; 2 * b + 1 < nm; b++)
+            for (int h = 0; h < 2; h++)
+                for (int m = 0; m < 2; m++)
+                    for (int q = 0; q < 2; q++)
+                        if (pos < ni) slot[2 * b + m][2 * h + q] = order[pos++];
+        if (nm % 2 != 0)
+            for (int q = 0; q < 4; q++)
+                if (pos < ni) slot[nm - 1][q] = order[pos++];
+        for (int o = 0; o < nm; o++) {
+            haspair[o] = false;
+            for (int q = 0; q < 4; q++) haspair[o] = haspair[o] || (slot[o][q] >= 0 && ib[slot[o][q]] >= 0);
+            int found = -1;
+            for (int g = 0; g < nsig && found < 0; g++) {
+                bool same = true;
+                for (int q = 0; q < 4; q++) {
+                    const int u = slot[o][q], v = slot[sigm[g]][q];
+                    same = same && (u < 0 ? -1 : key[u]) == (v < 0 ? -1 : key[v]);
+                }
+                if (same) found = g;
+            }
+            if (found < 0) {
+                found = nsig++;
+                sigm[found] = o;
+            }
+            sig[o] = found;
+        }
+    }
+};
+template <int R, bool ISO>
+struct F8PlanOf {
+    static constexpr F8Plan<R, ISO> value{};
+    static_assert(value.ni <= F8Plan<R, ISO>::kMaxItems && value.nm <= F8Plan<R, ISO>::kMaxM, "FP8 plan too large");
+};
+
 template <int I, int N, class F>
 __device__ __forceinline__ void mstatic_for(F &&f)
 {
@@ -188,6 +285,7 @@ template <class TD>
 struct DPlane;
 template <>
 struct DPlane<_Float16> {
+    static constexpr int XWE = XW, RYPE = RYP;  // LDS row pitch and rows per slice, in elements
     typedef half8 vec8;
     static __device__ __forceinline__ _Float16 from(float v) { return (_Float16)v; }
     static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
@@ -197,11 +295,23 @@ struct DPlane<_Float16> {
 };
 template <>
 struct DPlane<__bf16> {
+    static constexpr int XWE = XW, RYPE = RYP;
     typedef bf16x8 vec8;
     static __device__ __forceinline__ __bf16 from(float v) { return (__bf16)v; }
     static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+// FP8 e4m3 plane: 32 bytes per lane and operand, v_mfma_f32_16x16x128_f8f6f4
+// (format 0 = FP8 e4m3 for both operands, no block scales)
+template <>
+struct DPlane<f8_t> {
+    static constexpr int XWE = XW8, RYPE = RYP8;
+    typedef v8i vec8;
+    static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
+    {
+        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
     }
 };
 
@@ -230,6 +340,9 @@ struct MfmaArgs {
     int K;
     int min_lin;  // smallest linear search offset (<= 0): the tile's buffer ranges start there
     float one;    // 1.0 (a runtime value the compiler cannot fold)
+    // FP8 planes: FP8 e4m3 codes of band[p][i] * wscale
+    unsigned char band8[kMfmaMaxPatterns][8];
+    float wscale;
     // 16-bit storage: R and P are each followed, `shifted` elements further
     // on, by a copy shifted by one element (copy[j] = volume[j + 1])
     long shifted;
@@ -243,7 +356,7 @@ struct MfmaArgs {
 // two to four times as many.  16-bit storage therefore stages chunks of 8
 // columns (two units) in S8 slots of NTM chunks and any remainder as 4-column
 // chunks in S4 slots; FP32 storage stages 4-column chunks (16 bytes).
-template <int R, class TS>
+template <int R, class TS, class TD>
 struct MTile {
     static constexpr int RY = MTY + 2 * R, RZ = MTZ + 2 * R;
     static constexpr int UPR = 10;  // 4-column units per region row
@@ -261,11 +374,11 @@ struct MTile {
     static constexpr int CPT = S8 + S4;
     // staging slots past N8 (N4) write to their own 8 (4) elements behind the
     // plane, so every thread stages its chunks with no per-chunk branch
-    static constexpr int PLANE = RZ * RYP * XW;
+    static constexpr int PLANE = RZ * DPlane<TD>::RYPE * DPlane<TD>::XWE;
     static constexpr int SPARE8 = PLANE;
     static constexpr int SPARE4 = PLANE + 8 * (S8 * NTM - N8);
     static constexpr int LDS = SPARE4 + 4 * (S4 * NTM - N4);  // elements per buffer
-    static_assert(RY <= RYP, "patch radius too large for the padded slice");
+    static_assert(RY <= DPlane<TD>::RYPE, "patch radius too large for the padded slice");
     static_assert(UPR % 2 == 0, "8-column chunks must not straddle region rows");
 };
 
@@ -287,9 +400,10 @@ __device__ __forceinline__ int mxcd_swizzle(int wg, int nwg)
 // vector instructions per output and offset in the weight step (distance
 // scale, polynomial, cutoff test and select, the three sums)
 constexpr int kWeightValu = 13;
-// per staged unit of four values (difference, square, clamp; packed for FP16)
-template <class TS>
-constexpr int kStageValu = sizeof(TS) == 2 ? 6 : 12;
+// per staged unit of four values (difference, square, clamp; packed for FP16;
+// and the conversion for FP8 planes)
+template <class TS, class TD>
+constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 8 : sizeof(TS) == 2 ? 6 : 12;
 // per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
 // step, product, the sums)
 constexpr int kWeightValuPacked = 16;
@@ -453,6 +567,39 @@ struct Stager<W, _Float16, _Float16> {
     }
 };
 
+// FP16 storage and FP8 plane: packed half difference and square, clamped
+// below the largest value that the scale keeps under the FP8 maximum (448;
+// the conversion does not saturate), then converted two values per
+// instruction, scaled by S: the conversion divides by its scale operand,
+// here 1/S.
+typedef short short2v __attribute__((ext_vector_type(2)));
+template <int W>
+struct Stager<W, _Float16, f8_t> {
+    typedef half2v chunk[W / 2];
+    float inv;  // 1 / S, a power of two
+    half2v lim; // 448 / S
+    __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, sizeof(chunk)); }
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
+    {
+        load_chunk<W, _Float16>(&c[0], r, off, soff);
+    }
+    __device__ __forceinline__ void store(const chunk &pn, const chunk &b, f8_t *dst) const
+    {
+        unsigned words[W / 4];
+#pragma unroll
+        for (int h = 0; h < W / 4; h++) {
+            half2v d0 = pn[2 * h] - b[2 * h], d1 = pn[2 * h + 1] - b[2 * h + 1];
+            d0 = __builtin_elementwise_min(d0 * d0, lim);
+            d1 = __builtin_elementwise_min(d1 * d1, lim);
+            short2v r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(short2v{0, 0}, d0, inv, false);
+            r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, d1, inv, true);
+            __builtin_memcpy(&words[h], &r, 4);
+        }
+        // chunk columns are multiples of W in the row (W-byte aligned)
+        __builtin_memcpy(__builtin_assume_aligned(dst, W), words, W);
+    }
+};
+
 // 3 workgroups of 8 waves per CU (6 waves per SIMD) fit the LDS for R <= 2;
 // the register budget is capped to match.  FP32 storage holds twice the
 // staging registers and runs 2 workgroups per CU.
@@ -463,10 +610,12 @@ template <int R, class TS, class TD, int WFN, bool PRUNED, bool ISO>
 __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu<R, TS>(), mfma_waves_per_eu<R, TS>())))
 void filter_mfma(MfmaArgs<TS> a)
 {
-    using T = MTile<R, TS>;
+    using T = MTile<R, TS, TD>;
     using DP = DPlane<TD>;
     typedef typename DP::vec8 vec8;
     constexpr const MRows<R> &rows = MRowsOf<R>::value;
+    constexpr bool kF8 = std::is_same<TD, f8_t>::value;
+    constexpr const F8Plan<R, ISO> &pl = F8PlanOf<R, ISO>::value;
     __shared__ __attribute__((aligned(16))) TD lds[2][T::LDS];
 
     const int t = threadIdx.x;
@@ -482,7 +631,7 @@ void filter_mfma(MfmaArgs<TS> a)
 
     // Zero both buffers: the guard columns are never staged and must hold
     // zeros (they meet zero weights in the band, and 0 * NaN would poison).
-    for (int i = t; i < T::LDS; i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
+    for (int i = t; i < (int)(sizeof(lds) / 4); i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
 
     // Band operands, one per pattern (per pattern class when ISO) and kind.
     // Lane l holds entry k = 8*(l/16) + i of output column n = l % 16.  A
@@ -490,19 +639,56 @@ void filter_mfma(MfmaArgs<TS> a)
     // pair of one-tap rows reads columns x0 .. x0+15 of its first row for
     // k < 16 and of its second row for k >= 16, so output n takes k = n and
     // k = n + 16.
+    // FP8 planes: band operand g, lane group q holds slot q of an MFMA that
+    // uses it, bytes k = 0..31: a single row reads input column x0 - 4 + k
+    // (dx = k - 4 - n), a pair reads x0 + k of its first row (k < 16) and
+    // x0 + k - 16 of its second; the bytes are FP8 codes of the weights
+    // times the common scale a.wscale.
     const int n = lane & 15, q = lane >> 4;
-    constexpr int NB = ISO ? rows.nib : rows.nob;
+    constexpr int NB = kF8 ? pl.nsig : ISO ? rows.nib : rows.nob;
     vec8 bop[NB];
+    if constexpr (!kF8) {
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
-        const int p = ISO ? rows.ibpat[b] : rows.obpat[b];
-        const bool pair = (ISO ? rows.ibkind[b] : rows.obkind[b]) != 0;
+        for (int b = 0; b < NB; b++) {
+            const int p = ISO ? rows.ibpat[b] : rows.obpat[b];
+            const bool pair = (ISO ? rows.ibkind[b] : rows.obkind[b]) != 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int k = 8 * q + i, dx = k - 8 - n;
-            const float w = pair ? ((k & 15) == n ? a.band[p][R] : 0.f)
-                                 : ((dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f);
-            bop[b][i] = DP::from(w);
+            for (int i = 0; i < 8; i++) {
+                const int k = 8 * q + i, dx = k - 8 - n;
+                const float w = pair ? ((k & 15) == n ? a.band[p][R] : 0.f)
+                                     : ((dx >= -rows.pj[p] && dx <= rows.pj[p]) ? a.band[p][dx + R] : 0.f);
+                bop[b][i] = DP::from(w);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+#pragma unroll
+            for (int wd = 0; wd < 8; wd++) {
+                unsigned word = 0;
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) {
+                    const int it = pl.slot[pl.sigm[b]][qq];
+                    if (it < 0) continue;
+                    const int pa = rows.r[pl.ia[it]].pat;
+                    const int pp = ISO ? rows.crep[rows.cls[pa]] : pa;
+                    unsigned v = 0;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int k = 4 * wd + e;
+                        unsigned c = 0;
+                        if (pl.ib[it] >= 0) {
+                            c = (k & 15) == n ? a.band8[pp][R] : 0u;
+                        } else {
+                            const int dx = k - 4 - n;
+                            c = (dx >= -rows.pj[pp] && dx <= rows.pj[pp]) ? a.band8[pp][dx + R] : 0u;
+                        }
+                        v |= c << (8 * e);
+                    }
+                    word = q == qq ? v : word;
+                }
+                bop[b][wd] = (int)word;
+            }
         }
     }
 
@@ -528,7 +714,9 @@ void filter_mfma(MfmaArgs<TS> a)
         const int row = u / T::UPR, col = 4 * (u % T::UPR);
         const int ry = row % T::RY, rz = row / T::RY;
         co = (int)(rz * sp + (long)ry * g.nxp + col);
-        lo = (rz * RYP + ry) * XW + 4 + col;
+        // 16-bit planes: LDS column c holds x = X0 - 8 + c; FP8 planes:
+        // x = X0 - 4 + c (8-byte aligned chunk stores, 16-byte aligned row windows)
+        lo = (rz * DP::RYPE + ry) * DP::XWE + (kF8 ? 0 : 4) + col;
     };
 #pragma unroll
     for (int c = 0; c < S8; c++) {
@@ -576,6 +764,36 @@ void filter_mfma(MfmaArgs<TS> a)
     const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
     // D operand: lane l reads row n = l % 16, columns 16*xh + 8*q .. +7.
     const int aoff = ((2 * gz + (n >> 3) + R) * RYP + ((n & 7) + R)) * XW + 16 * xh + 8 * q;
+    // FP8 planes: lane l reads 32 bytes of row n of the item in slot q of
+    // each MFMA: a single row from LDS column 16*xh (x0 - 4), a pair 16
+    // bytes from column 16*xh + 4 (x0) of each of its two rows; an empty
+    // slot reads the lane's own row against zero band bytes.
+    constexpr int NM8 = kF8 ? pl.nm : 1;
+    int f8a[NM8], f8b[NM8];
+    if constexpr (kF8) {
+        const int mrow = ((2 * gz + (n >> 3) + R) * RYP8 + ((n & 7) + R)) * XW8 + 16 * xh;
+        auto roff = [&](int r) __attribute__((always_inline)) { return (rows.r[r].dz * RYP8 + rows.r[r].dy) * XW8; };
+#pragma unroll
+        for (int o = 0; o < NM8; o++) {
+            int oa = mrow, ob = mrow + 16;
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) {
+                const int it = pl.slot[o][qq];
+                int ua = mrow, ub = mrow + 16;
+                if (it >= 0 && pl.ib[it] >= 0) {
+                    ua = mrow + 4 + roff(pl.ia[it]);
+                    ub = mrow + 4 + roff(pl.ib[it]);
+                } else if (it >= 0) {
+                    ua = mrow + roff(pl.ia[it]);
+                    ub = ua + 16;
+                }
+                oa = q == qq ? ua : oa;
+                ob = q == qq ? ub : ob;
+            }
+            f8a[o] = oa;
+            f8b[o] = ob;
+        }
+    }
     // for a pair of one-tap rows: columns x0 + 8 (q % 2) of the first row
     // (q < 2) or of the second (q >= 2)
     int poff[rows.npd > 0 ? rows.npd : 1];
@@ -586,11 +804,35 @@ void filter_mfma(MfmaArgs<TS> a)
     // orders those like floats and needs no NaN canonicalisation
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
-    const float ms = mm * weight_scale_factor<WFN>();
+    // FP8 planes hold D * S, S a power of two per tile that puts the
+    // distance at the weight cutoff (|a| = 3.57, d = 3.57 / |mm|) near 14,
+    // inside the FP8 normal range [2^-6, 448]; the band holds the weights
+    // times a.wscale, so the accumulator is d * S * wscale.
+    float dinv = 1.f;  // 1 / (S * wscale)
+    float sinv = 1.f;  // 1 / S
+    if constexpr (kF8) {
+        const int zm = Z0 + MTZ / 2 < g.nz ? Z0 + MTZ / 2 : g.nz - 1;
+        const float smid = a.sigma[zm];
+        const float amid = __builtin_fabsf((-1.f / (smid * smid * a.beta)) * a.mscale) * 4.f;
+        // exponent of amid from its bits (0 and subnormals give -127,
+        // infinities and NaN 128; both clamp)
+        int e = (int)((__builtin_bit_cast(unsigned, amid) >> 23) & 0xffu) - 127;
+        e = e < -30 ? -30 : e > 30 ? 30 : e;
+        sinv = __builtin_bit_cast(float, (unsigned)(127 - e) << 23);
+        dinv = sinv / a.wscale;
+    }
+    const float ms = mm * weight_scale_factor<WFN>() * dinv;
     constexpr bool kPacked = std::is_same<TS, _Float16>::value && WFN == 0;
     ushort2w wmax2[2] = {{0, 0}, {0, 0}};
-    const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f));
+    const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f) * dinv);
     const half2w msu = {msu1, msu1};
+    if constexpr (kF8) {
+        const _Float16 l16 = (_Float16)(448.f * sinv);
+        stg8.inv = sinv;
+        stg8.lim = half2v{l16, l16};
+        stg4.inv = sinv;
+        stg4.lim = half2v{l16, l16};
+    }
 
     // Buffer resources per volume and tile, starting at the smallest search
     // offset: a search offset is then the uniform (scalar) part of the
@@ -738,41 +980,69 @@ void filter_mfma(MfmaArgs<TS> a)
         load_r(rout, l0);
         const TD *L = rd + aoff;
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-        // A operands in flight: KD reads ahead of the MFMA that consumes them,
-        // so the chain waits on LDS latency once rather than per row
-        constexpr int NR = rows.nops;
-        constexpr int KD = NR < 2 ? NR : 2;
-        auto a_of = [&](auto oi) __attribute__((always_inline)) {
-            constexpr int o = decltype(oi)::value;
-            constexpr const MRows<R> &rw = MRowsOf<R>::value;
-            constexpr int off = MRows<R>::lds_off(rw.r[rw.opa[o]]);
-            if constexpr (rw.opb[o] >= 0)
-                return *reinterpret_cast<const vec8 *>(rd + poff[rw.opd[o]] + off);
-            else
-                return *reinterpret_cast<const vec8 *>(L + off);
-        };
-        vec8 ab[KD];
-        mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = a_of(ri); });
-        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
-            constexpr int r = decltype(ri)::value;
-            constexpr int b = ISO ? MRowsOf<R>::value.ibop[r] : MRowsOf<R>::value.obop[r];
-            acc = DP::mfma(bop[b], ab[r % KD], acc);
-            if constexpr (r + KD < NR) ab[r % KD] = a_of(std::integral_constant<int, r + KD>{});
-        });
-        accumulate(din, rin);
-        stage(st);
-        // order: the R load, KD reads, then per MFMA the next A read and a
-        // share of the vector work (none right after the first MFMA); the
-        // staging writes and the next loads follow the chain
-        constexpr int NV = ((kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu) + kStageValu<TS> * (2 * T::S8 + T::S4) + NR - 2) /
-                           (NR - 1);
-        __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
-        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
-            __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
-            if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
-            if constexpr (decltype(ri)::value >= 1) __builtin_amdgcn_sched_group_barrier(0x0002, NV, 0);
-        });
+        if constexpr (!kF8) {
+            // A operands in flight: KD reads ahead of the MFMA that consumes
+            // them, so the chain waits on LDS latency once rather than per row
+            constexpr int NR = rows.nops;
+            constexpr int KD = NR < 2 ? NR : 2;
+            auto a_of = [&](auto oi) __attribute__((always_inline)) {
+                constexpr int o = decltype(oi)::value;
+                constexpr const MRows<R> &rw = MRowsOf<R>::value;
+                constexpr int off = MRows<R>::lds_off(rw.r[rw.opa[o]]);
+                if constexpr (rw.opb[o] >= 0)
+                    return *reinterpret_cast<const vec8 *>(rd + poff[rw.opd[o]] + off);
+                else
+                    return *reinterpret_cast<const vec8 *>(L + off);
+            };
+            vec8 ab[KD];
+            mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = a_of(ri); });
+            mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+                constexpr int r = decltype(ri)::value;
+                constexpr int b = ISO ? MRowsOf<R>::value.ibop[r] : MRowsOf<R>::value.obop[r];
+                acc = DP::mfma(bop[b], ab[r % KD], acc);
+                if constexpr (r + KD < NR) ab[r % KD] = a_of(std::integral_constant<int, r + KD>{});
+            });
+            accumulate(din, rin);
+            stage(st);
+            // order: the R load, KD reads, then per MFMA the next A read and
+            // a share of the vector work (none right after the first MFMA);
+            // the staging writes and the next loads follow the chain
+            constexpr int NV = ((kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu) +
+                                kStageValu<TS, TD> * (2 * T::S8 + T::S4) + NR - 2) /
+                               (NR - 1);
+            __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
+            mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+                __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
+                if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
+                if constexpr (decltype(ri)::value >= 1) __builtin_amdgcn_sched_group_barrier(0x0002, NV, 0);
+            });
+        } else {
+            // FP8: NM8 MFMAs of four slots; a D operand is two 16-byte
+            // halves (4-byte aligned when the MFMA holds a pair)
+            constexpr int KD = NM8 < 2 ? NM8 : 2;
+            auto d_of = [&](auto oi) __attribute__((always_inline)) {
+                constexpr int o = decltype(oi)::value;
+                vec8 d;
+                if constexpr (F8PlanOf<R, ISO>::value.haspair[o]) {
+                    __builtin_memcpy(&d, __builtin_assume_aligned(rd + f8a[o], 4), 16);
+                    __builtin_memcpy(reinterpret_cast<char *>(&d) + 16, __builtin_assume_aligned(rd + f8b[o], 4), 16);
+                } else {
+                    __builtin_memcpy(&d, __builtin_assume_aligned(rd + f8a[o], 16), 16);
+                    __builtin_memcpy(reinterpret_cast<char *>(&d) + 16, __builtin_assume_aligned(rd + f8b[o], 16), 16);
+                }
+                return d;
+            };
+            vec8 ab[KD];
+            mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = d_of(ri); });
+            mstatic_for<0, NM8>([&](auto ri) __attribute__((always_inline)) {
+                constexpr int r = decltype(ri)::value;
+                acc = DP::mfma(bop[F8PlanOf<R, ISO>::value.sig[r]], ab[r % KD], acc);
+                if constexpr (r + KD < NM8) ab[r % KD] = d_of(std::integral_constant<int, r + KD>{});
+            });
+            accumulate(din, rin);
+            stage(st);
+        }
         // the loads below overwrite the chunks the staging has just read
         __builtin_amdgcn_sched_barrier(0);
         load_chunks(l2);
@@ -896,6 +1166,56 @@ bool mfma_bands_r(const int *dxyz, const float *w, int ntap, float (*band)[8])
     return count == ntap;
 }
 
+// OCP FP8 e4m3 (bias 7, no infinities, 448 the largest finite value):
+// round to nearest even, values >= 0.
+unsigned char e4m3_code(float x)
+{
+    if (!(x > 0.f)) return 0;
+    if (x >= 448.f) return 0x7e;
+    int e = 0;
+    (void)std::frexp(x, &e);
+    int E = e - 1;  // x = 1.f * 2^E
+    if (E < -6) {   // subnormal: multiples of 2^-9
+        const long r = std::lrint(std::ldexp((double)x, 9));
+        return (unsigned char)(r >= 8 ? 0x08 : r);
+    }
+    long m = std::lrint((std::ldexp((double)x, -E) - 1.0) * 8.0);
+    if (m == 8) {
+        m = 0;
+        E++;
+    }
+    const int code = ((E + 7) << 3) | (int)m;
+    return (unsigned char)(code > 0x7e ? 0x7e : code);
+}
+float e4m3_value(unsigned char c)
+{
+    const int E = (c >> 3) & 15, m = c & 7;
+    return E == 0 ? std::ldexp((float)m, -9) : std::ldexp(1.f + m / 8.f, E - 7);
+}
+// The common factor in [1, 2) that brings the band weights closest to FP8
+// values (largest relative rounding error over the weights), and the codes.
+float fp8_bands(const float (*band)[8], int npat, unsigned char (*codes)[8])
+{
+    float best = 1.f, err_best = 1e30f;
+    for (int i = 0; i < 4096; i++) {
+        const float c = 1.f + i / 4096.f;
+        float err = 0.f;
+        for (int p = 0; p < npat; p++)
+            for (int k = 0; k < 8; k++)
+                if (band[p][k] > 0.f) {
+                    const float v = band[p][k] * c;
+                    err = std::max(err, std::fabs(e4m3_value(e4m3_code(v)) / v - 1.f));
+                }
+        if (err < err_best) {
+            err_best = err;
+            best = c;
+        }
+    }
+    for (int p = 0; p < kMfmaMaxPatterns; p++)
+        for (int k = 0; k < 8; k++) codes[p][k] = p < npat ? e4m3_code(band[p][k] * best) : 0;
+    return best;
+}
+
 template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
@@ -934,6 +1254,10 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
     a.nty = (g.ny + MTY - 1) / MTY;
     a.ntz = (g.nz + MTZ - 1) / MTZ;
     if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
+    a.wscale = 1.f;
+    std::memset(a.band8, 0, sizeof(a.band8));
+    if constexpr (std::is_same<TD, f8_t>::value)
+        a.wscale = fp8_bands(a.band, MRowsOf<R>::value.npat, a.band8);
     const long nwg = (long)a.ntx * a.nty * a.ntz;
     if (ol && (ol->ntiles != nwg || !ol->lists || !ol->counts || ol->K < 1)) return hipErrorInvalidValue;
     // patterns of one class carry identical bands when the weights are
@@ -964,6 +1288,9 @@ hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t ele
     case FORM_FP16:
         return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                          inv_scale, search, n_search, min_lin, taps, g, ol, st);
+    case FORM_FP16_FP8:
+        return launch_mfma_t<R, _Float16, f8_t, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
+                                                     inv_scale, search, n_search, min_lin, taps, g, ol, st);
     case FORM_BF16:
         return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                      inv_scale, search, n_search, min_lin, taps, g, ol, st);
@@ -1017,6 +1344,7 @@ hipError_t launch_pad_normalise(int form, const float *in, void *out, long shift
                            shift, scale);
         break;
     case FORM_FP16:
+    case FORM_FP16_FP8:
         hipLaunchKernelGGL(pad_normalise_kernel<_Float16>, dim3(blocks), dim3(256), 0, st, in, (_Float16 *)out,
                            shifted, g, shift, scale);
         break;

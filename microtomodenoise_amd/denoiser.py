@@ -65,7 +65,7 @@ class DenoiseParameters:
     cpu: CPU = field(default_factory=CPU)
     gpu: GPU = field(default_factory=GPU)
     # build-specific choices
-    precision: str = "fp32"  # fp32 (parity), fp32_fast, bf16, fp16
+    precision: str = "fp32"  # fp32 (parity), fp32_fast, bf16, fp16, fp16_fp8
     weight_fn: str = "pade"  # pade (reference CPU), exp (reference GPU)
     sample_seed: int = 0
     kernel_select: int = 0  # 0 tiled where available, 1 per-voxel
@@ -98,7 +98,7 @@ class DenoiseParameters:
         p.sample_seed = int(self.sample_seed)
         p.cpu_blockwise = int(self.cpu.blockwise)
         p.precision = {"fp32": capi.PREC_FP32, "fp32_fast": capi.PREC_FP32_FAST, "bf16": capi.PREC_BF16,
-                       "fp16": capi.PREC_FP16}[self.precision]
+                       "fp16": capi.PREC_FP16, "fp16_fp8": capi.PREC_FP16_FP8}[self.precision]
         p.weight_fn = {"pade": capi.WEIGHT_PADE, "exp": capi.WEIGHT_EXP}[self.weight_fn]
         p.kernel_select = int(self.kernel_select)
         return p

@@ -125,10 +125,10 @@ def test_adam_update(gpu_available):
 # weights, so one FP16 rounding that moves a distance across the weight
 # cutoff moves the output further than in the full search: the FP16 form's
 # maximum is looser, its rms is not.
-PRUNED_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.5, 0.006)}
+PRUNED_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.5, 0.006), "fp16_fp8": (1.0, 0.02)}
 
 
-@pytest.mark.parametrize("form", ["fp32_fast", "fp16"])
+@pytest.mark.parametrize("form", ["fp32_fast", "fp16", "fp16_fp8"])
 def test_pruned_lists_match_oracle_per_tile(gpu_available, form):
     spec = ph.PhantomSpec((64, 24, 16), seed=71, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8, streaks=3,
                           streak_len=(5, 15))
@@ -175,7 +175,8 @@ def test_pruned_lists_match_oracle_per_tile(gpu_available, form):
     assert sep > 10 * rms
 
 
-def test_proposal_fallback_audit_and_topk(gpu_available):
+@pytest.mark.parametrize("form", ["fp16", "fp16_fp8"])
+def test_proposal_fallback_audit_and_topk(gpu_available, form):
     spec = ph.PhantomSpec((64, 32, 24), seed=81, radii=(2, 7), streaks=3, streak_len=(5, 15))
     raw = ph.make(spec)
     nz, ny, nx = raw.shape
@@ -186,10 +187,10 @@ def test_proposal_fallback_audit_and_topk(gpu_available):
     net = PR.OffsetProposal(ns, seed=3)
     rd = torch.from_numpy(raw).cuda()
     probs = net.predict(rd.data_ptr(), [nx, ny, nz])
-    print(f"[gpu] proposal modes {raw.shape} K={K} of {ns}", flush=True)
+    print(f"[gpu] proposal modes {form} {raw.shape} K={K} of {ns}", flush=True)
 
     def run(mode):
-        g, _ = make_gpu((nx, ny, nz), rs, rp, precision="fp16")
+        g, _ = make_gpu((nx, ny, nz), rs, rp, precision=form)
         g.load_raw(raw)
         g.set_sigma(sig)
         if mode == "full":
