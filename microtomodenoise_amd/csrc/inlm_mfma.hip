@@ -403,9 +403,9 @@ constexpr int kWeightValu = 13;
 // per staged unit of four values (difference, square, clamp; packed for FP16;
 // and the conversion for FP8 planes)
 template <class TS, class TD>
-constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 8 : sizeof(TS) == 2 ? 6 : 12;
+constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 6 : sizeof(TS) == 2 ? 6 : 12;
 // per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
-// step, product, the sums)
+// step, product, the sums), degree-7 polynomial
 constexpr int kWeightValuPacked = 16;
 constexpr float kPadePoly[8] = {0.1679810881614685f,  0.30033349990844727f,  0.2661350965499878f,
                                 0.1601015031337738f,  0.06991086900234222f,  0.024921204894781113f,
@@ -420,19 +420,29 @@ constexpr _Float16 kPadePolyU[8] = {(_Float16)0.02496337890625f,  (_Float16)0.11
                                     (_Float16)0.1502685546875f,   (_Float16)0.2890625f,
                                     (_Float16)-0.055084228515625f, (_Float16)0.625f,
                                     (_Float16)-0.426025390625f,   (_Float16)0.2763671875f};
+// The same as a degree-5 polynomial, for the FP8-plane form whose FP8
+// distances and weights already carry errors near 1e-2: at most 3.0e-3 of
+// the weight (1.1e-3 rms) evaluated in FP16, two instructions fewer per pair.
+constexpr _Float16 kPadePolyU5[6] = {(_Float16)0.024932861328125f, (_Float16)0.11859130859375f,
+                                     (_Float16)0.098388671875f,    (_Float16)0.54541015625f,
+                                     (_Float16)-0.50146484375f,    (_Float16)0.712890625f};
 typedef _Float16 half2w __attribute__((ext_vector_type(2)));
 typedef unsigned short ushort2w __attribute__((ext_vector_type(2)));
 // the weights of two pairs of outputs, the two polynomial chains in lockstep
+// (DEG 7: kPadePolyU, DEG 5: kPadePolyU5)
+template <int DEG>
 __device__ __forceinline__ void packed_weights(half2w d0, half2w d1, half2w ms, half2w &w0, half2w &w1)
 {
+    static_assert(DEG == 7 || DEG == 5, "polynomial degree");
+    constexpr const _Float16 *C = DEG == 7 ? kPadePolyU : kPadePolyU5;
     const half2w zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
     half2w u0 = __builtin_elementwise_fma(d0, ms, one), u1 = __builtin_elementwise_fma(d1, ms, one);
     u0 = __builtin_elementwise_min(__builtin_elementwise_max(u0, zero), one);
     u1 = __builtin_elementwise_min(__builtin_elementwise_max(u1, zero), one);
-    half2w p0 = {kPadePolyU[7], kPadePolyU[7]}, p1 = p0;
+    half2w p0 = {C[DEG], C[DEG]}, p1 = p0;
 #pragma unroll
-    for (int j = 6; j >= 0; j--) {
-        const half2w c = {kPadePolyU[j], kPadePolyU[j]};
+    for (int j = DEG - 1; j >= 0; j--) {
+        const half2w c = {C[j], C[j]};
         p0 = __builtin_elementwise_fma(p0, u0, c);
         p1 = __builtin_elementwise_fma(p1, u1, c);
     }
@@ -567,36 +577,46 @@ struct Stager<W, _Float16, _Float16> {
     }
 };
 
-// FP16 storage and FP8 plane: packed half difference and square, clamped
-// below the largest value that the scale keeps under the FP8 maximum (448;
-// the conversion does not saturate), then converted two values per
-// instruction, scaled by S: the conversion divides by its scale operand,
-// here 1/S.
+// FP16 storage and FP8 plane.  With f = 2^m per tile the plane holds
+// D * S, S = 256 f^2: the staging forms (P(y+s) f - P(y) f) with one packed
+// FMA (the tile's own chunk is kept as -P f), squares it clamped to [0, 1]
+// (the instruction's clamp; values beyond and NaN become 1 and 0), and
+// converts two values per instruction times 256 (the conversion divides by
+// its scale operand, 1/256): no value exceeds the FP8 maximum of 448, which
+// the conversion does not saturate.
 typedef short short2v __attribute__((ext_vector_type(2)));
 template <int W>
 struct Stager<W, _Float16, f8_t> {
     typedef half2v chunk[W / 2];
-    float inv;  // 1 / S, a power of two
-    half2v lim; // 448 / S
+    half2v f;  // the tile's 2^m
     __device__ __forceinline__ void load(chunk &c, const _Float16 *p) const { __builtin_memcpy(&c[0], p, sizeof(chunk)); }
     __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
     {
         load_chunk<W, _Float16>(&c[0], r, off, soff);
     }
+    // b: the tile's own chunk times -f
     __device__ __forceinline__ void store(const chunk &pn, const chunk &b, f8_t *dst) const
     {
+        const half2v zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
         unsigned words[W / 4];
 #pragma unroll
         for (int h = 0; h < W / 4; h++) {
-            half2v d0 = pn[2 * h] - b[2 * h], d1 = pn[2 * h + 1] - b[2 * h + 1];
-            d0 = __builtin_elementwise_min(d0 * d0, lim);
-            d1 = __builtin_elementwise_min(d1 * d1, lim);
-            short2v r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(short2v{0, 0}, d0, inv, false);
-            r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, d1, inv, true);
+            const half2v d0 = __builtin_elementwise_fma(pn[2 * h], f, b[2 * h]);
+            const half2v d1 = __builtin_elementwise_fma(pn[2 * h + 1], f, b[2 * h + 1]);
+            const half2v e0 = __builtin_elementwise_min(__builtin_elementwise_max(d0 * d0, zero), one);
+            const half2v e1 = __builtin_elementwise_min(__builtin_elementwise_max(d1 * d1, zero), one);
+            short2v r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(short2v{0, 0}, e0, 1.f / 256.f, false);
+            r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, e1, 1.f / 256.f, true);
             __builtin_memcpy(&words[h], &r, 4);
         }
         // chunk columns are multiples of W in the row (W-byte aligned)
         __builtin_memcpy(__builtin_assume_aligned(dst, W), words, W);
+    }
+    // the tile's own chunk as kept in LDS
+    __device__ __forceinline__ void own(chunk &b) const
+    {
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) b[h] = -(b[h] * f);
     }
 };
 
@@ -692,6 +712,25 @@ void filter_mfma(MfmaArgs<TS> a)
         }
     }
 
+    // FP8 planes hold D * S with S = 256 f^2, f = 2^m per tile, chosen so
+    // that the distance at the weight cutoff of the tile's middle slice
+    // (|a| = 3.57, d = 3.57 / |mm|) lands in [2, 8] after scaling: inside
+    // the FP8 normal range [2^-6, 448] and below the clamp at 256 times the
+    // smallest patch weight.  The band holds the weights times a.wscale, so
+    // the accumulator is d * S * wscale.
+    float f8f = 1.f;
+    if constexpr (kF8) {
+        const int zm = Z0 + MTZ / 2 < g.nz ? Z0 + MTZ / 2 : g.nz - 1;
+        const float smid = a.sigma[zm];
+        const float x = __builtin_fabsf((-1.f / (smid * smid * a.beta)) * a.mscale) * (8.f / 3.56648f / 256.f);
+        // exponent of x from its bits (0 and subnormals give -127, infinity
+        // and NaN 128; both clamp), S / 256 = 4^m <= x
+        int e = (int)((__builtin_bit_cast(unsigned, x) >> 23) & 0xffu) - 127;
+        e = e < -24 ? -24 : e > 16 ? 16 : e;
+        const int m = e >> 1;
+        f8f = __builtin_bit_cast(float, (unsigned)(127 + m) << 23);
+    }
+
     // Staging chunks: 8 or 4 columns of one region row.  Region row (rz, ry)
     // is slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
     const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
@@ -707,6 +746,11 @@ void filter_mfma(MfmaArgs<TS> a)
     __shared__ __attribute__((aligned(16))) chunk4 p0s4[S4 > 0 ? S4 * NTM : 1];
     ST8 stg8;
     ST4 stg4;
+    if constexpr (kF8) {
+        const _Float16 fh = (_Float16)f8f;
+        stg8.f = half2v{fh, fh};
+        stg4.f = half2v{fh, fh};
+    }
     int coff8[S8 > 0 ? S8 : 1], loff8[S8 > 0 ? S8 : 1], coff4[S4 > 0 ? S4 : 1], loff4[S4 > 0 ? S4 : 1];
     // unit u (4 columns) of the region: its volume offset from Pt and its
     // LDS element offset
@@ -729,6 +773,7 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk8 b;
         stg8.load(b, Pt + coff8[c]);
+        if constexpr (kF8) stg8.own(b);
         __builtin_memcpy(&p0s8[cs], b, sizeof(b));
     }
 #pragma unroll
@@ -742,6 +787,7 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk4 b;
         stg4.load(b, Pt + coff4[c]);
+        if constexpr (kF8) stg4.own(b);
         __builtin_memcpy(&p0s4[cs], b, sizeof(b));
     }
 
@@ -804,35 +850,13 @@ void filter_mfma(MfmaArgs<TS> a)
     // orders those like floats and needs no NaN canonicalisation
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
-    // FP8 planes hold D * S, S a power of two per tile that puts the
-    // distance at the weight cutoff (|a| = 3.57, d = 3.57 / |mm|) near 14,
-    // inside the FP8 normal range [2^-6, 448]; the band holds the weights
-    // times a.wscale, so the accumulator is d * S * wscale.
-    float dinv = 1.f;  // 1 / (S * wscale)
-    float sinv = 1.f;  // 1 / S
-    if constexpr (kF8) {
-        const int zm = Z0 + MTZ / 2 < g.nz ? Z0 + MTZ / 2 : g.nz - 1;
-        const float smid = a.sigma[zm];
-        const float amid = __builtin_fabsf((-1.f / (smid * smid * a.beta)) * a.mscale) * 4.f;
-        // exponent of amid from its bits (0 and subnormals give -127,
-        // infinities and NaN 128; both clamp)
-        int e = (int)((__builtin_bit_cast(unsigned, amid) >> 23) & 0xffu) - 127;
-        e = e < -30 ? -30 : e > 30 ? 30 : e;
-        sinv = __builtin_bit_cast(float, (unsigned)(127 - e) << 23);
-        dinv = sinv / a.wscale;
-    }
+    float dinv = 1.f;  // FP8 planes: 1 / (S * wscale)
+    if constexpr (kF8) dinv = 1.f / (256.f * f8f * f8f * a.wscale);
     const float ms = mm * weight_scale_factor<WFN>() * dinv;
     constexpr bool kPacked = std::is_same<TS, _Float16>::value && WFN == 0;
     ushort2w wmax2[2] = {{0, 0}, {0, 0}};
     const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f) * dinv);
     const half2w msu = {msu1, msu1};
-    if constexpr (kF8) {
-        const _Float16 l16 = (_Float16)(448.f * sinv);
-        stg8.inv = sinv;
-        stg8.lim = half2v{l16, l16};
-        stg4.inv = sinv;
-        stg4.lim = half2v{l16, l16};
-    }
 
     // Buffer resources per volume and tile, starting at the smallest search
     // offset: a search offset is then the uniform (scalar) part of the
@@ -901,7 +925,7 @@ void filter_mfma(MfmaArgs<TS> a)
     auto accumulate = [&](const floatx4 &d, const RVals &rv) __attribute__((always_inline)) {
         if constexpr (kPacked) {
             half2w w[2];
-            packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
+            packed_weights<kF8 ? 5 : 7>(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
                            __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
