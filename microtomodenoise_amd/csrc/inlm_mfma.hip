@@ -53,6 +53,8 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 // OCP FP8 e4m3 D-plane element
 typedef unsigned char f8_t;
+// FP8-plane plans: at most this many MFMAs per offset and band operands
+constexpr int kF8MaxM = 8, kF8MaxSig = 8;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -269,6 +271,7 @@ template <int R, bool ISO>
 struct F8PlanOf {
     static constexpr F8Plan<R, ISO> value{};
     static_assert(value.ni <= F8Plan<R, ISO>::kMaxItems && value.nm <= F8Plan<R, ISO>::kMaxM, "FP8 plan too large");
+    static_assert(value.nm <= kF8MaxM && value.nsig <= kF8MaxSig, "FP8 plan exceeds the argument tables");
 };
 
 template <int I, int N, class F>
@@ -340,8 +343,12 @@ struct MfmaArgs {
     int K;
     int min_lin;  // smallest linear search offset (<= 0): the tile's buffer ranges start there
     float one;    // 1.0 (a runtime value the compiler cannot fold)
-    // FP8 planes: FP8 e4m3 codes of band[p][i] * wscale
-    unsigned char band8[kMfmaMaxPatterns][8];
+    // FP8 planes: band operand g, lane group q: a 48-byte Toeplitz template
+    // of FP8 e4m3 codes (the weights times wscale) that lane n reads from
+    // byte 16 - n; and the LDS byte offsets, relative to the lane's own row,
+    // of the two 16-byte halves of slot q of MFMA o
+    unsigned char f8tmpl[kF8MaxSig][4][48];
+    int f8offa[kF8MaxM][4], f8offb[kF8MaxM][4];
     float wscale;
     // 16-bit storage: R and P are each followed, `shifted` elements further
     // on, by a copy shifted by one element (copy[j] = volume[j + 1])
@@ -681,33 +688,19 @@ void filter_mfma(MfmaArgs<TS> a)
             }
         }
     } else {
+        // the template row of this lane's group, read from byte 16 - n
+        // (4-byte loads at any byte offset: the argument block is read as
+        // bytes and packed)
+        const unsigned char *tb = &a.f8tmpl[0][0][0];
 #pragma unroll
         for (int b = 0; b < NB; b++) {
+            const unsigned char *row = tb + (b * 4 + q) * 48 + 16 - n;
 #pragma unroll
             for (int wd = 0; wd < 8; wd++) {
-                unsigned word = 0;
+                unsigned w = 0;
 #pragma unroll
-                for (int qq = 0; qq < 4; qq++) {
-                    const int it = pl.slot[pl.sigm[b]][qq];
-                    if (it < 0) continue;
-                    const int pa = rows.r[pl.ia[it]].pat;
-                    const int pp = ISO ? rows.crep[rows.cls[pa]] : pa;
-                    unsigned v = 0;
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const int k = 4 * wd + e;
-                        unsigned c = 0;
-                        if (pl.ib[it] >= 0) {
-                            c = (k & 15) == n ? a.band8[pp][R] : 0u;
-                        } else {
-                            const int dx = k - 4 - n;
-                            c = (dx >= -rows.pj[pp] && dx <= rows.pj[pp]) ? a.band8[pp][dx + R] : 0u;
-                        }
-                        v |= c << (8 * e);
-                    }
-                    word = q == qq ? v : word;
-                }
-                bop[b][wd] = (int)word;
+                for (int e = 0; e < 4; e++) w |= (unsigned)row[4 * wd + e] << (8 * e);
+                bop[b][wd] = (int)w;
             }
         }
     }
@@ -818,26 +811,10 @@ void filter_mfma(MfmaArgs<TS> a)
     int f8a[NM8], f8b[NM8];
     if constexpr (kF8) {
         const int mrow = ((2 * gz + (n >> 3) + R) * RYP8 + ((n & 7) + R)) * XW8 + 16 * xh;
-        auto roff = [&](int r) __attribute__((always_inline)) { return (rows.r[r].dz * RYP8 + rows.r[r].dy) * XW8; };
 #pragma unroll
         for (int o = 0; o < NM8; o++) {
-            int oa = mrow, ob = mrow + 16;
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) {
-                const int it = pl.slot[o][qq];
-                int ua = mrow, ub = mrow + 16;
-                if (it >= 0 && pl.ib[it] >= 0) {
-                    ua = mrow + 4 + roff(pl.ia[it]);
-                    ub = mrow + 4 + roff(pl.ib[it]);
-                } else if (it >= 0) {
-                    ua = mrow + roff(pl.ia[it]);
-                    ub = ua + 16;
-                }
-                oa = q == qq ? ua : oa;
-                ob = q == qq ? ub : ob;
-            }
-            f8a[o] = oa;
-            f8b[o] = ob;
+            f8a[o] = mrow + a.f8offa[o][q];
+            f8b[o] = mrow + a.f8offb[o][q];
         }
     }
     // for a pair of one-tap rows: columns x0 + 8 (q % 2) of the first row
@@ -1240,6 +1217,49 @@ float fp8_bands(const float (*band)[8], int npat, unsigned char (*codes)[8])
     return best;
 }
 
+// FP8 planes: per band operand and lane group the Toeplitz template (byte
+// 16 - n + k of the template is byte k of lane n: a single row of half-width
+// j puts its codes at 20 + dx, a pair of one-tap rows its two codes at 16
+// and 32), and per MFMA and lane group the LDS offsets of the two 16-byte
+// halves relative to the lane's own row (a single row from its window start
+// at x0 - 4, a pair from x0 in each of its rows; an empty slot reads the
+// lane's own row).
+template <int R, bool ISO, class A>
+void fp8_tables_plan(const unsigned char (*codes)[8], A &a)
+{
+    constexpr const MRows<R> &rows = MRowsOf<R>::value;
+    constexpr const F8Plan<R, ISO> &pl = F8PlanOf<R, ISO>::value;
+    for (int g = 0; g < pl.nsig; g++)
+        for (int q = 0; q < 4; q++) {
+            unsigned char *t = a.f8tmpl[g][q];
+            const int it = pl.slot[pl.sigm[g]][q];
+            if (it < 0) continue;
+            const int pa = rows.r[pl.ia[it]].pat, pp = ISO ? rows.crep[rows.cls[pa]] : pa;
+            if (pl.ib[it] >= 0) {
+                const int pb = rows.r[pl.ib[it]].pat, qb = ISO ? rows.crep[rows.cls[pb]] : pb;
+                t[16] = codes[pp][R];
+                t[32] = codes[qb][R];
+            } else {
+                for (int dx = -rows.pj[pp]; dx <= rows.pj[pp]; dx++) t[20 + dx] = codes[pp][dx + R];
+            }
+        }
+    auto roff = [&](int r) { return (rows.r[r].dz * RYP8 + rows.r[r].dy) * XW8; };
+    for (int o = 0; o < pl.nm; o++)
+        for (int q = 0; q < 4; q++) {
+            const int it = pl.slot[o][q];
+            int oa = 0, ob = 16;
+            if (it >= 0 && pl.ib[it] >= 0) {
+                oa = 4 + roff(pl.ia[it]);
+                ob = 4 + roff(pl.ib[it]);
+            } else if (it >= 0) {
+                oa = roff(pl.ia[it]);
+                ob = oa + 16;
+            }
+            a.f8offa[o][q] = oa;
+            a.f8offb[o][q] = ob;
+        }
+}
+
 template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
@@ -1279,9 +1299,11 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
     a.ntz = (g.nz + MTZ - 1) / MTZ;
     if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
     a.wscale = 1.f;
-    std::memset(a.band8, 0, sizeof(a.band8));
-    if constexpr (std::is_same<TD, f8_t>::value)
-        a.wscale = fp8_bands(a.band, MRowsOf<R>::value.npat, a.band8);
+    std::memset(a.f8tmpl, 0, sizeof(a.f8tmpl));
+    std::memset(a.f8offa, 0, sizeof(a.f8offa));
+    std::memset(a.f8offb, 0, sizeof(a.f8offb));
+    unsigned char codes[kMfmaMaxPatterns][8] = {};
+    if constexpr (std::is_same<TD, f8_t>::value) a.wscale = fp8_bands(a.band, MRowsOf<R>::value.npat, codes);
     const long nwg = (long)a.ntx * a.nty * a.ntz;
     if (ol && (ol->ntiles != nwg || !ol->lists || !ol->counts || ol->K < 1)) return hipErrorInvalidValue;
     // patterns of one class carry identical bands when the weights are
@@ -1290,6 +1312,11 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
     bool iso = true;
     for (int p = 0; p < rows.npat; p++)
         for (int i = 0; i < 8; i++) iso = iso && a.band[p][i] == a.band[rows.crep[rows.cls[p]]][i];
+    if constexpr (std::is_same<TD, f8_t>::value) {
+        // the tables follow the plan of the kernel launched below
+        if (iso) fp8_tables_plan<R, true>(codes, a);
+        else fp8_tables_plan<R, false>(codes, a);
+    }
     const dim3 grid((unsigned)nwg), block(NTM);
     if (ol && iso) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, true>), grid, block, 0, st, a);
     else if (ol) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, false>), grid, block, 0, st, a);
