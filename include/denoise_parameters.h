@@ -49,7 +49,7 @@ struct DenoiseParameters {
         int memory_buffer = 0;
         bool blockwise_host = false;
         // build-specific
-        std::string precision = "fp32";  // "fp32" (parity), "fp32_fast", "bf16", "fp16"
+        std::string precision = "fp32";  // "fp32" (parity), "fp32_fast", "bf16", "fp16", "fp16_fp8"
         std::string weight_fn = "pade";  // "pade" (reference CPU) or "exp" (reference GPU)
         uint64_t sample_seed = 0;
     };

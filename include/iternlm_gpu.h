@@ -111,6 +111,7 @@ public:
         p.precision = pr == "fp32_fast" ? INLM_PREC_FP32_FAST
                       : pr == "bf16"    ? INLM_PREC_BF16
                       : pr == "fp16"    ? INLM_PREC_FP16
+                      : pr == "fp16_fp8" ? INLM_PREC_FP16_FP8
                                         : INLM_PREC_FP32;
         p.weight_fn = d.gpu.weight_fn == "exp" ? INLM_WEIGHT_EXP : INLM_WEIGHT_PADE;
         return p;
