@@ -587,10 +587,10 @@ struct Stager<W, _Float16, _Float16> {
 // FP16 storage and FP8 plane.  With f = 2^m per tile the plane holds
 // D * S, S = 256 f^2: the staging forms (P(y+s) f - P(y) f) with one packed
 // FMA (the tile's own chunk is kept as -P f), squares it clamped to [0, 1]
-// (the instruction's clamp; values beyond and NaN become 1 and 0), and
-// converts two values per instruction times 256 (the conversion divides by
-// its scale operand, 1/256): no value exceeds the FP8 maximum of 448, which
-// the conversion does not saturate.
+// (the instruction's clamp: larger values become 1), and converts two values
+// per instruction times 256 (the conversion divides by its scale operand,
+// 1/256): no value exceeds the FP8 maximum of 448, which the conversion does
+// not saturate.
 typedef short short2v __attribute__((ext_vector_type(2)));
 template <int W>
 struct Stager<W, _Float16, f8_t> {
@@ -708,9 +708,11 @@ void filter_mfma(MfmaArgs<TS> a)
     // FP8 planes hold D * S with S = 256 f^2, f = 2^m per tile, chosen so
     // that the distance at the weight cutoff of the tile's middle slice
     // (|a| = 3.57, d = 3.57 / |mm|) lands in [2, 8] after scaling: inside
-    // the FP8 normal range [2^-6, 448] and below the clamp at 256 times the
-    // smallest patch weight.  The band holds the weights times a.wscale, so
-    // the accumulator is d * S * wscale.
+    // the FP8 normal range [2^-6, 448], and below what one clamped tap adds
+    // (256 times its weight; 10 for the smallest r = 2 weight, 0.04), so a
+    // patch pair with one saturated tap stays beyond the cutoff.  The band
+    // holds the weights times a.wscale, so the accumulator is
+    // d * S * wscale.
     float f8f = 1.f;
     if constexpr (kF8) {
         const int zm = Z0 + MTZ / 2 < g.nz ? Z0 + MTZ / 2 : g.nz - 1;
@@ -725,7 +727,8 @@ void filter_mfma(MfmaArgs<TS> a)
     }
 
     // Staging chunks: 8 or 4 columns of one region row.  Region row (rz, ry)
-    // is slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c.
+    // is slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c
+    // (16-bit planes) or x = X0 - 4 + c (FP8 planes).
     const TS *Pt = a.P + ((long)(Z0 - R + g.pz) * sp + (long)(Y0 - R + g.py) * g.nxp + (X0 - 4 + g.px));
     constexpr int W8 = sizeof(TS) == 2 ? 8 : 4;
     using ST8 = Stager<W8, TS, TD>;

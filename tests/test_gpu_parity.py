@@ -222,3 +222,27 @@ def test_fast_forms_vs_fp64_oracle(gpu_available, form, rp, rs, nx):
     print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
     bmax, brms = FAST_BUDGET[form]
     assert mx <= bmax and rms <= brms, (mx, rms)
+
+
+@pytest.mark.parametrize("form", ["fp32_fast", "fp16", "fp16_fp8"])
+def test_fast_forms_anisotropic_patch(gpu_available, form):
+    # z anisotropy makes the weights depend on (dx^2 + dy^2, dz^2) rather
+    # than on the distance alone: the per-pattern (not per-class) operands
+    spec = ph.PhantomSpec((40, 30, 20), seed=52, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8)
+    raw = ph.make(spec)
+    nz, ny, nx = raw.shape
+    rs, rp = (3, 3, 2), (2, 2, 2)
+    sig = np.full(nz, 700.0, np.float32)
+    print(f"[gpu] anisotropic patch, fast form {form}", flush=True)
+    g, _ = make_gpu((nx, ny, nz), rs, rp, precision=form, z_anisotropy=2.0)
+    g.set_sigma(sig)
+    g.Run_GaussianNoise(1, raw)
+    got = g.get_result().astype(np.float64)
+    g.free_device()
+    p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False, z_anisotropy=2.0)
+    ref = O.filter_iteration(raw, raw, sig.astype(np.float64), p, precision="f64")
+    err = np.abs(got - ref) / spec.noise_sigma
+    mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
+    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
+    bmax, brms = FAST_BUDGET[form]
+    assert mx <= bmax and rms <= brms, (mx, rms)
