@@ -1206,13 +1206,23 @@ float e4m3_value(unsigned char c)
     const int E = (c >> 3) & 15, m = c & 7;
     return E == 0 ? std::ldexp((float)m, -9) : std::ldexp(1.f + m / 8.f, E - 7);
 }
-// The common factor in [1, 2) that brings the band weights closest to FP8
-// values (largest relative rounding error over the weights), and the codes.
+// The common factor in [1, 2), a multiple of 1/512, that brings the band
+// weights closest to FP8 values (largest relative rounding error over the
+// weights), and the codes.
 float fp8_bands(const float (*band)[8], int npat, unsigned char (*codes)[8])
 {
+    // one entry per host thread: a handle launches with the same bands every
+    // iteration
+    thread_local float last_band[kMfmaMaxPatterns][8];
+    thread_local int last_npat = -1;
+    thread_local float last_c = 1.f;
     float best = 1.f, err_best = 1e30f;
-    for (int i = 0; i < 4096; i++) {
-        const float c = 1.f + i / 4096.f;
+    if (npat == last_npat && std::memcmp(last_band, band, sizeof(float) * 8 * (size_t)npat) == 0) {
+        best = last_c;
+        err_best = 0.f;
+    }
+    for (int i = 0; i < 512 && err_best > 0.f; i++) {
+        const float c = 1.f + i / 512.f;
         float err = 0.f;
         for (int p = 0; p < npat; p++)
             for (int k = 0; k < 8; k++)
@@ -1227,6 +1237,9 @@ float fp8_bands(const float (*band)[8], int npat, unsigned char (*codes)[8])
     }
     for (int p = 0; p < kMfmaMaxPatterns; p++)
         for (int k = 0; k < 8; k++) codes[p][k] = p < npat ? e4m3_code(band[p][k] * best) : 0;
+    std::memcpy(last_band, band, sizeof(float) * 8 * (size_t)npat);
+    last_npat = npat;
+    last_c = best;
     return best;
 }
 
