@@ -43,6 +43,8 @@ WORKLOADS = {
     "cfg1": ("cfg1", 5, 1, "128x128x64 uint16 phantom (seed 0), patch r=1 (7 taps), search r=5 (514 offsets)"),
     "cfg3": ("cfg3", 7, 3, "1024x1024x512 float32 three-phase phantom (seed 2), patch r=3 (115 taps), "
                            "search r=7 (1326 offsets)"),
+    "cfg5": ("cfg5", 7, 2, "2048x2048x256 uint16 four-level phantom (seed 4, noise 2500, blur 1.0, 1000 streaks), "
+                           "patch r=2 (33 taps), search r=7 (1326 offsets)"),
 }
 
 
