@@ -222,13 +222,24 @@ struct F8Plan {
             order[i] = i;
             for (int k = 0; k < ni; k++) cnt[i] += key[k] == key[i] ? 1 : 0;
         }
-        // insertion sort by (descending count, key, item)
+        // insertion sort by (descending count, key, dy^2, dy, dz of the
+        // first row): consecutive items share an MFMA in lane groups 0/1 or
+        // 2/3, which LDS serves in the same cycles; rows with equal dy whose
+        // dz differ by 2 lie a multiple of 16 bank groups apart (48-byte
+        // rows, 24 rows per slice), so such pairs read without conflicts
+        auto rank = [&](int it) {
+            const MRow &w = rows.r[ia[it]];
+            return ((w.dy * w.dy) * 16 + (w.dy + 8)) * 16 + (w.dz + 8);
+        };
         for (int i = 1; i < ni; i++) {
             const int v = order[i];
             int j = i - 1;
             while (j >= 0) {
                 const int u = order[j];
-                const bool before = cnt[v] != cnt[u] ? cnt[v] > cnt[u] : (key[v] != key[u] ? key[v] < key[u] : v < u);
+                const bool before = cnt[v] != cnt[u]   ? cnt[v] > cnt[u]
+                                    : key[v] != key[u] ? key[v] < key[u]
+                                    : rank(v) != rank(u) ? rank(v) < rank(u)
+                                                         : v < u;
                 if (!before) break;
                 order[j + 1] = u;
                 j--;
