@@ -255,9 +255,10 @@ def main():
         outs[f] = run_form(f, vol, nx, ny, nz, rs, rp, s0, sig1, args.steps, args.warmup, log)
     V = nx * ny * nz
     clean_t = torch.from_numpy(clean).to("cuda")
-    # PSNR peak: the uint16 range for phantoms on that scale, 1 for the
-    # unit-range three-phase phantom (PSNR differences do not depend on it)
-    peak = 1.0 if max(ph.CONFIGS[cfg].levels) <= 1.0 else 65535.0
+    # PSNR peak: 65535 for phantoms on the uint16 scale, the clean volume's
+    # range for the unit-range three-phase phantom (PSNR differences do not
+    # depend on it)
+    peak = float(clean.max() - clean.min()) if max(ph.CONFIGS[cfg].levels) <= 1.0 else 65535.0
     psnr = lambda m: 10.0 * np.log10(peak * peak / m)  # noqa: E731
 
     def mse_clean(out):
