@@ -423,45 +423,49 @@ constexpr int kWeightValu = 13;
 template <class TS, class TD>
 constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 6 : sizeof(TS) == 2 ? 6 : 12;
 // per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
-// step, product, the sums), degree-7 polynomial; degree 5
-constexpr int kWeightValuPacked = 16;
-constexpr int kWeightValuPacked5 = 14;
+// step, the sums), degree-7 polynomial; degree 5
+constexpr int kWeightValuPacked = 15;
+constexpr int kWeightValuPacked5 = 13;
 constexpr float kPadePoly[8] = {0.1679810881614685f,  0.30033349990844727f,  0.2661350965499878f,
                                 0.1601015031337738f,  0.06991086900234222f,  0.024921204894781113f,
                                 0.008454217575490475f, 0.002158894669264555f};
 // FP16 storage form: the weights of two outputs per instruction in packed
 // FP16.  u = a / (2h) + 1 = d m / 3.56648 + 1, clamped to [0, 1] (u = 0 at
-// and beyond the cutoff), and the Pade form as a degree-7 minimax polynomial
-// in u with FP16 coefficients, times a step that is 1 for u >= 2^-14 and 0
-// at u = 0.  Evaluated in FP16 it deviates from the rational form by at most
-// 1.1e-3 of the weight (2.6e-4 rms), most of it from rounding u to FP16.
-constexpr _Float16 kPadePolyU[8] = {(_Float16)0.02496337890625f,  (_Float16)0.11541748046875f,
-                                    (_Float16)0.1502685546875f,   (_Float16)0.2890625f,
-                                    (_Float16)-0.055084228515625f, (_Float16)0.625f,
-                                    (_Float16)-0.426025390625f,   (_Float16)0.2763671875f};
-// The same as a degree-5 polynomial, for the FP8-plane form whose FP8
-// distances and weights already carry errors near 1e-2: at most 3.0e-3 of
-// the weight (1.1e-3 rms) evaluated in FP16, two instructions fewer per pair.
-constexpr _Float16 kPadePolyU5[6] = {(_Float16)0.024932861328125f, (_Float16)0.11859130859375f,
-                                     (_Float16)0.098388671875f,    (_Float16)0.54541015625f,
-                                     (_Float16)-0.50146484375f,    (_Float16)0.712890625f};
+// and beyond the cutoff).  The weight is computed divided by its value at
+// the cutoff, c0 = 0.025 (the sums and the maximum weight all scale alike,
+// so the result is unchanged): w / c0 = Q(u) u + s(u), with Q a polynomial
+// with FP16 coefficients and s a step that is 1 for u >= 2^-14 and 0 at
+// u = 0.  Dividing by c0 turns the step's product into the last Horner
+// addend, one instruction fewer per pair.  The coefficients minimise the
+// largest deviation of the FP16 evaluation over every FP16 u in [2^-13, 1].
+// Degree 7 (Q of degree 6): on a uniform grid of a (a > -0.999 h) at most
+// 1.6e-3 of the weight from the rational form (2.5e-4 rms), most of it from
+// rounding u to FP16.
+constexpr _Float16 kPadePolyQ[7] = {(_Float16)4.4921875f,   (_Float16)7.85546875f, (_Float16)0.77392578125f,
+                                    (_Float16)28.015625f,   (_Float16)-18.640625f, (_Float16)14.3125f,
+                                    (_Float16)2.189453125f};
+// Degree 5 (Q of degree 4), for the FP8-plane form whose FP8 distances and
+// weights already carry errors near 1e-2: at most 1.7e-3 (3.3e-4 rms), with
+// c0 = 0.0248, two instructions fewer per pair.
+constexpr _Float16 kPadePolyQ5[5] = {(_Float16)5.2734375f, (_Float16)-0.86474609375f, (_Float16)37.53125f,
+                                     (_Float16)-40.03125f, (_Float16)37.40625f};
 typedef _Float16 half2w __attribute__((ext_vector_type(2)));
 typedef unsigned short ushort2w __attribute__((ext_vector_type(2)));
-// the weights of two pairs of outputs, the two polynomial chains in lockstep
-// (DEG 7: kPadePolyU, DEG 5: kPadePolyU5)
+// the weights (divided by c0) of two pairs of outputs, the two polynomial
+// chains in lockstep (DEG 7: kPadePolyQ, DEG 5: kPadePolyQ5)
 template <int DEG>
 __device__ __forceinline__ void packed_weights(half2w d0, half2w d1, half2w ms, half2w &w0, half2w &w1)
 {
     static_assert(DEG == 7 || DEG == 5, "polynomial degree");
-    constexpr const _Float16 *C = DEG == 7 ? kPadePolyU : kPadePolyU5;
+    constexpr const _Float16 *Q = DEG == 7 ? kPadePolyQ : kPadePolyQ5;
     const half2w zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
     half2w u0 = __builtin_elementwise_fma(d0, ms, one), u1 = __builtin_elementwise_fma(d1, ms, one);
     u0 = __builtin_elementwise_min(__builtin_elementwise_max(u0, zero), one);
     u1 = __builtin_elementwise_min(__builtin_elementwise_max(u1, zero), one);
-    half2w p0 = {C[DEG], C[DEG]}, p1 = p0;
+    half2w p0 = {Q[DEG - 1], Q[DEG - 1]}, p1 = p0;
 #pragma unroll
-    for (int j = DEG - 1; j >= 0; j--) {
-        const half2w c = {C[j], C[j]};
+    for (int j = DEG - 2; j >= 0; j--) {
+        const half2w c = {Q[j], Q[j]};
         p0 = __builtin_elementwise_fma(p0, u0, c);
         p1 = __builtin_elementwise_fma(p1, u1, c);
     }
@@ -469,8 +473,8 @@ __device__ __forceinline__ void packed_weights(half2w d0, half2w d1, half2w ms, 
     half2w s0 = u0 * big, s1 = u1 * big;
     s0 = __builtin_elementwise_min(__builtin_elementwise_max(s0, zero), one);
     s1 = __builtin_elementwise_min(__builtin_elementwise_max(s1, zero), one);
-    w0 = p0 * s0;
-    w1 = p1 * s1;
+    w0 = __builtin_elementwise_fma(p0, u0, s0);
+    w1 = __builtin_elementwise_fma(p1, u1, s1);
 }
 
 template <int WFN>
@@ -624,7 +628,10 @@ struct Stager<W, _Float16, f8_t> {
             const half2v d1 = __builtin_elementwise_fma(pn[2 * h + 1], f, b[2 * h + 1]);
             const half2v e0 = __builtin_elementwise_min(__builtin_elementwise_max(d0 * d0, zero), one);
             const half2v e1 = __builtin_elementwise_min(__builtin_elementwise_max(d1 * d1, zero), one);
-            short2v r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(short2v{0, 0}, e0, 1.f / 256.f, false);
+            // the conversion keeps the other half of its destination; the
+            // first one writes over e0's own register (its high half is
+            // replaced by the second), which saves a register clear
+            short2v r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(__builtin_bit_cast(short2v, e0), e0, 1.f / 256.f, false);
             r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, e1, 1.f / 256.f, true);
             __builtin_memcpy(&words[h], &r, 4);
         }
@@ -748,10 +755,18 @@ void filter_mfma(MfmaArgs<TS> a)
     typedef typename ST8::chunk chunk8;
     typedef typename ST4::chunk chunk4;
     constexpr int S8 = T::S8, S4 = T::S4;
-    // the chunks at the tile's own positions stay fixed over the offsets and
-    // live in LDS, one slot per (thread, chunk)
-    __shared__ __attribute__((aligned(16))) chunk8 p0s8[S8 > 0 ? S8 * NTM : 1];
-    __shared__ __attribute__((aligned(16))) chunk4 p0s4[S4 > 0 ? S4 * NTM : 1];
+    // the chunks at the tile's own positions stay fixed over the offsets.
+    // The FP16 and FP8-plane forms with isotropic patches up to radius 2 keep
+    // them in registers (their LDS reads sat on each step's critical path: 5%
+    // of the FP8-plane pass); the other instances, which would spill in the
+    // loop, keep them in LDS, one slot per (thread, chunk)
+    constexpr bool kP0Reg = WFN == 0 && ISO && R <= 2 &&
+                            (kF8 ? (!PRUNED || R < 2) : (std::is_same<TS, _Float16>::value && std::is_same<TD, _Float16>::value));
+    constexpr int P0L8 = kP0Reg ? 1 : (S8 > 0 ? S8 * NTM : 1), P0L4 = kP0Reg ? 1 : (S4 > 0 ? S4 * NTM : 1);
+    __shared__ __attribute__((aligned(16))) chunk8 p0s8[P0L8];
+    __shared__ __attribute__((aligned(16))) chunk4 p0s4[P0L4];
+    chunk8 p0r8[kP0Reg && S8 > 0 ? S8 : 1];
+    chunk4 p0r4[kP0Reg && S4 > 0 ? S4 : 1];
     ST8 stg8;
     ST4 stg4;
     if constexpr (kF8) {
@@ -782,7 +797,10 @@ void filter_mfma(MfmaArgs<TS> a)
         chunk8 b;
         stg8.load(b, Pt + coff8[c]);
         if constexpr (kF8) stg8.own(b);
-        __builtin_memcpy(&p0s8[cs], b, sizeof(b));
+        if constexpr (kP0Reg)
+            __builtin_memcpy(&p0r8[c], b, sizeof(b));
+        else
+            __builtin_memcpy(&p0s8[cs], b, sizeof(b));
     }
 #pragma unroll
     for (int c = 0; c < S4; c++) {
@@ -796,7 +814,10 @@ void filter_mfma(MfmaArgs<TS> a)
         chunk4 b;
         stg4.load(b, Pt + coff4[c]);
         if constexpr (kF8) stg4.own(b);
-        __builtin_memcpy(&p0s4[cs], b, sizeof(b));
+        if constexpr (kP0Reg)
+            __builtin_memcpy(&p0r4[c], b, sizeof(b));
+        else
+            __builtin_memcpy(&p0s4[cs], b, sizeof(b));
     }
 
     // This lane's outputs.  The MFMA forms the transposed block,
@@ -893,9 +914,9 @@ void filter_mfma(MfmaArgs<TS> a)
     };
     auto stage = [&](TD *buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < S8; c++) stg8.store(pn8[c], p0s8[t + c * NTM], buf + loff8[c]);
+        for (int c = 0; c < S8; c++) stg8.store(pn8[c], kP0Reg ? p0r8[c] : p0s8[t + c * NTM], buf + loff8[c]);
 #pragma unroll
-        for (int c = 0; c < S4; c++) stg4.store(pn4[c], p0s4[t + c * NTM], buf + loff4[c]);
+        for (int c = 0; c < S4; c++) stg4.store(pn4[c], kP0Reg ? p0r4[c] : p0s4[t + c * NTM], buf + loff4[c]);
     };
     // R at this lane's four outputs shifted by one search offset, in the
     // stored type
@@ -1051,20 +1072,27 @@ void filter_mfma(MfmaArgs<TS> a)
             };
             vec8 ab[KD];
             mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = d_of(ri); });
+            if constexpr (kP0Reg) stage(st);
             accumulate(din, rin);
             mstatic_for<0, NM8>([&](auto ri) __attribute__((always_inline)) {
                 constexpr int r = decltype(ri)::value;
                 acc = DP::mfma(bop[F8PlanOf<R, ISO>::value.sig[r]], ab[r % KD], acc);
                 if constexpr (r + KD < NM8) ab[r % KD] = d_of(std::integral_constant<int, r + KD>{});
             });
-            stage(st);
-            // order: the first KD operand reads, then the weights of the
-            // previous offset (registers only) while they land, then the
-            // first MFMA; with three MFMAs per offset there is little else
-            // to cover the LDS latency (2.4% faster than leaving it to the
-            // compiler)
+            if constexpr (!kP0Reg) stage(st);
+            // order: the first KD operand reads; while they land, the staging
+            // of the next plane when its inputs are all in registers (its two
+            // LDS writes then overlap the MFMA chain instead of preceding the
+            // barrier) and the weights of the previous offset; then the first
+            // MFMA.  With three MFMAs per offset there is little else to cover
+            // the LDS latency.
             constexpr int NVA = kPacked ? 2 * kWeightValuPacked5 : 4 * kWeightValu;
+            constexpr int NVS = kStageValu<TS, TD> * (2 * T::S8 + T::S4);
             __builtin_amdgcn_sched_group_barrier(0x0100, 2 * KD, 0);
+            if constexpr (kP0Reg) {
+                __builtin_amdgcn_sched_group_barrier(0x0002, NVS, 0);
+                __builtin_amdgcn_sched_group_barrier(0x0200, T::S8 + T::S4, 0);
+            }
             __builtin_amdgcn_sched_group_barrier(0x0002, NVA, 0);
             __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
         }
