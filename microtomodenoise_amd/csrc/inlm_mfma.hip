@@ -899,8 +899,14 @@ void filter_mfma(MfmaArgs<TS> a)
     auto rsrc_r = [&](int lin) __attribute__((always_inline)) {
         return __builtin_amdgcn_make_buffer_rsrc(const_cast<TS *>(odd_of(lin) ? Rb1 : Rb), (short)0, nR, kRsrcWord3);
     };
+    // the smallest offset and the runtime 1.0, read here: an argument first
+    // read inside the loop would make every step wait for all outstanding LDS
+    // work (scalar argument loads and LDS share one wait counter)
+    const int min_lin = a.min_lin;
+    const float one = a.one;
+    asm volatile("" ::"s"(min_lin), "s"(one));
     auto soff_of = [&](int lin) __attribute__((always_inline)) {
-        return (unsigned)((lin - a.min_lin) * (int)sizeof(TS));
+        return (unsigned)((lin - min_lin) * (int)sizeof(TS));
     };
     chunk8 pn8[S8 > 0 ? S8 : 1];
     chunk4 pn4[S4 > 0 ? S4 : 1];
@@ -945,7 +951,7 @@ void filter_mfma(MfmaArgs<TS> a)
                 const float wi = (float)w[i >> 1][i & 1];
                 fv[i] = __builtin_fmaf(wi, (float)rv.v[i], fv[i]);
                 // times a runtime 1.0, so the sum stays one mixed-precision FMA
-                fw[i] = __builtin_fmaf(wi, a.one, fw[i]);
+                fw[i] = __builtin_fmaf(wi, one, fw[i]);
             }
             // weights are non-negative: their FP16 bit patterns order like them
 #pragma unroll
