@@ -34,7 +34,7 @@ struct inlm_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     inlm::Geometry geo{};
     inlm::FilterGeom fg{};
-    std::vector<inlm::SearchEntry> search;
+    std::vector<inlm::SearchEntry> search, search_fast;
     std::vector<int> tap_dxyz;
     inlm::TapTable taps{};
     int rp_tiled = 0;
@@ -51,6 +51,9 @@ struct inlm_handle {
     bool fast = false;
     double shift = 0.0, scale = 1.0;
     inlm::SearchEntry *search_dev = nullptr;
+    // the same offsets in the order the matrix-core kernels sweep the whole
+    // search (the lists of the pruned forms index search_dev)
+    inlm::SearchEntry *search_fast_dev = nullptr;
     int min_lin = 0;  // smallest linear search offset
     int64_t *samples_dev = nullptr;
     double *std_dev = nullptr;
@@ -341,6 +344,25 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
     if ((e = hipMemcpyAsync(h->search_dev, h->search.data(), sizeof(inlm::SearchEntry) * h->search.size(),
                             hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
+    // Full-search order of the matrix-core forms: per search slice, the
+    // offsets whose 16-bit loads use one volume copy (even dx with an even
+    // row pitch) before those that use the other, each group row by row.
+    // Consecutive steps then read one copy at regions two columns apart, so
+    // the L1 cache keeps serving them; the reference order alternates the
+    // two copies at every step.  The sums are order-independent up to FP32
+    // rounding.
+    h->search_fast = h->search;
+    std::stable_sort(h->search_fast.begin(), h->search_fast.end(), [](const inlm::SearchEntry &a, const inlm::SearchEntry &b) {
+        if (a.dz != b.dz) return a.dz < b.dz;
+        if ((a.lin & 1) != (b.lin & 1)) return (a.lin & 1) < (b.lin & 1);
+        if (a.dy != b.dy) return a.dy < b.dy;
+        return a.dx < b.dx;
+    });
+    if ((e = hipMalloc(&h->search_fast_dev, sizeof(inlm::SearchEntry) * h->search_fast.size())) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMalloc(search)"));
+    if ((e = hipMemcpyAsync(h->search_fast_dev, h->search_fast.data(), sizeof(inlm::SearchEntry) * h->search_fast.size(),
+                            hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(hip_fail(nullptr, e, "sync"));
     (void)st;
     *out = h;
@@ -359,6 +381,7 @@ inlm_status inlm_destroy(inlm_handle *h)
     (void)hipFree(h->result);
     (void)hipFree(h->sigma);
     (void)hipFree(h->search_dev);
+    (void)hipFree(h->search_fast_dev);
     (void)hipFree(h->samples_dev);
     (void)hipFree(h->std_dev);
     (void)hipFree(h->lists_dev);
@@ -461,8 +484,8 @@ inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, f
         HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems, h->shifted,
                                             h->raw, h->result, h->sigma, h->p.beta,
                                             (float)(1.0 / (h->scale * h->scale)), h->shift, 1.0 / h->scale,
-                                            h->search_dev, ns, h->min_lin, h->taps, h->fg, h->stream,
-                                            pruned ? &ol : nullptr));
+                                            h->search_dev, h->search_fast_dev, ns, h->min_lin, h->taps,
+                                            h->fg, h->stream, pruned ? &ol : nullptr));
     } else {
         HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad,
                                        static_cast<const float *>(P), h->result, h->sigma, h->p.beta, h->search_dev,

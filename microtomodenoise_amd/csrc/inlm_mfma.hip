@@ -338,7 +338,8 @@ struct MfmaArgs {
     const float *raw;  // unpadded original
     float *out;
     const float *sigma;
-    const SearchEntry *search;
+    const SearchEntry *search;       // the search table; lists index it
+    const SearchEntry *search_full;  // the same offsets in full-search order
     int n_search;
     float beta;
     float mscale;      // 1 / scale^2: distances in stored units -> data units
@@ -985,7 +986,9 @@ void filter_mfma(MfmaArgs<TS> a)
                 return a.search[i < (unsigned)a.n_search ? i : 0u].lin;
             }
         }
-        return a.search[k].lin;
+        // the whole search (also a pruned tile's fallback) in one order, so
+        // a tile that falls back sums exactly like the full-search kernel
+        return a.search_full[k].lin;
     };
 
     // Software pipeline over the offsets (nS >= 1).  Step k runs the
@@ -1334,8 +1337,8 @@ void fp8_tables_plan(const unsigned char (*codes)[8], A &a)
 template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
-                         const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
-                         const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
+                         const SearchEntry *search, const SearchEntry *search_full, int n_search, int min_lin,
+                         const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
     if constexpr (sizeof(TS) == 2) {
         // 16-bit loads stay 4-byte aligned only with an even row pitch and
@@ -1359,6 +1362,7 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
     a.out = out;
     a.sigma = sigma;
     a.search = search;
+    a.search_full = search_full;
     a.n_search = n_search;
     a.beta = beta;
     a.mscale = mscale;
@@ -1400,22 +1404,26 @@ template <int R, int WFN>
 hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw,
                             float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
-                            const SearchEntry *search, int n_search, int min_lin, const TapTable &taps,
-                            const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
+                            const SearchEntry *search, const SearchEntry *search_full, int n_search, int min_lin,
+                            const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
         return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                      inv_scale, search, n_search, min_lin, taps, g, ol, st);
+                                                      inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
+                                                      st);
     case FORM_FP16:
         return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                         inv_scale, search, n_search, min_lin, taps, g, ol, st);
+                                                         inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
+                                                      st);
     case FORM_FP16_FP8:
         return launch_mfma_t<R, _Float16, f8_t, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, n_search, min_lin, taps, g, ol, st);
+                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
+                                                      st);
     case FORM_BF16:
         return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, n_search, min_lin, taps, g, ol, st);
+                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
+                                                      st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1435,16 +1443,16 @@ bool mfma_taps_supported(int rp, const int *dxyz, const float *w, int n)
 
 hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, const void *Ps, size_t elems,
                               long shifted, const float *raw, float *out, const float *sigma, float beta, float mscale,
-                              double shift, double inv_scale, const SearchEntry *search, int n_search,
-                              int min_lin, const TapTable &taps, const FilterGeom &g, hipStream_t st,
-                              const OffsetLists *ol)
+                              double shift, double inv_scale, const SearchEntry *search,
+                              const SearchEntry *search_full, int n_search, int min_lin, const TapTable &taps,
+                              const FilterGeom &g, hipStream_t st, const OffsetLists *ol)
 {
     if (min_lin > 0) return hipErrorInvalidValue;
 #define INLM_MF(RR)                                                                                               \
     return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, min_lin, taps, g, ol, st)                                 \
+                                                    search, search_full, n_search, min_lin, taps, g, ol, st)                                 \
                           : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, n_search, min_lin, taps, g, ol, st)
+                                                    search, search_full, n_search, min_lin, taps, g, ol, st)
     switch (rp) {
     case 1: INLM_MF(1);
     case 2: INLM_MF(2);
