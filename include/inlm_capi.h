@@ -53,8 +53,9 @@ typedef enum { INLM_NOISE_GLOBAL = 0, INLM_NOISE_ZADAPTIVE = 1, INLM_NOISE_MANUA
  *   FP32_FAST -- FP32 storage, fused multiply-add stencil and reciprocal-based
  *                Pade; differs from FP32 by rounding only.
  *   BF16/FP16 -- R and P stored in 16 bits after the affine map
- *                (v - shift) * scale; differences, sums and accumulators in
- *                FP32, normalisation in FP64.
+ *                (v - shift) * scale; squared differences in FP16 (BF16
+ *                values are widened to FP16 first), patch sums, weight sums
+ *                and accumulators in FP32, normalisation in FP64.
  *   FP16_FP8  -- FP16 storage as FP16, squared-difference planes in FP8
  *                e4m3 scaled per tile by a power of two, patch weights in
  *                FP8 times a common factor; distance sums in FP32. */

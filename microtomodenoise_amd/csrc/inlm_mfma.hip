@@ -422,7 +422,7 @@ constexpr int kWeightValu = 13;
 // per staged unit of four values (difference, square, clamp; packed for FP16;
 // and the conversion for FP8 planes)
 template <class TS, class TD>
-constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 6 : sizeof(TS) == 2 ? 6 : 12;
+constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 6 : std::is_same<TS, __bf16>::value ? 12 : sizeof(TS) == 2 ? 6 : 12;
 // per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
 // step, the sums), degree-7 polynomial; degree 5
 constexpr int kWeightValuPacked = 15;
@@ -598,6 +598,41 @@ struct Stager<W, _Float16, _Float16> {
             out[h] = __builtin_elementwise_min(d * d, big);
         }
         store_dplane<W>(dst, out);
+    }
+};
+
+// BF16 storage and FP16 plane: each pair of stored values is widened to
+// FP32 (a shift and a mask) and packed to FP16, exactly for the normalised
+// data's range; the difference, square and clamp then run as in the FP16
+// form.  The tile's own chunk is converted once (own) and kept as FP16.
+template <int W>
+struct Stager<W, __bf16, _Float16> {
+    typedef unsigned chunk[W / 2];  // two BF16 values per word (FP16 pairs after own)
+    __device__ __forceinline__ void load(chunk &c, const __bf16 *p) const { __builtin_memcpy(&c[0], p, sizeof(chunk)); }
+    __device__ __forceinline__ void load(chunk &c, rsrc_t r, unsigned off, unsigned soff) const
+    {
+        load_chunk<W, __bf16>(&c[0], r, off, soff);
+    }
+    static __device__ __forceinline__ half2v widen(unsigned v)
+    {
+        const float lo = __builtin_bit_cast(float, v << 16), hi = __builtin_bit_cast(float, v & 0xffff0000u);
+        return __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(lo, hi));
+    }
+    __device__ __forceinline__ void store(const chunk &pn, const chunk &b, _Float16 *dst) const
+    {
+        const half2v big = {(_Float16)65504.f, (_Float16)65504.f};
+        half2v out[W / 2];
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) {
+            const half2v d = widen(pn[h]) - __builtin_bit_cast(half2v, b[h]);
+            out[h] = __builtin_elementwise_min(d * d, big);
+        }
+        store_dplane<W>(dst, out);
+    }
+    __device__ __forceinline__ void own(chunk &b) const
+    {
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) b[h] = __builtin_bit_cast(unsigned, widen(b[h]));
     }
 };
 
@@ -797,7 +832,7 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk8 b;
         stg8.load(b, Pt + coff8[c]);
-        if constexpr (kF8) stg8.own(b);
+        if constexpr (kF8 || std::is_same<TS, __bf16>::value) stg8.own(b);
         if constexpr (kP0Reg)
             __builtin_memcpy(&p0r8[c], b, sizeof(b));
         else
@@ -814,7 +849,7 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk4 b;
         stg4.load(b, Pt + coff4[c]);
-        if constexpr (kF8) stg4.own(b);
+        if constexpr (kF8 || std::is_same<TS, __bf16>::value) stg4.own(b);
         if constexpr (kP0Reg)
             __builtin_memcpy(&p0r4[c], b, sizeof(b));
         else
@@ -867,7 +902,7 @@ void filter_mfma(MfmaArgs<TS> a)
     float dinv = 1.f;  // FP8 planes: 1 / (S * wscale)
     if constexpr (kF8) dinv = 1.f / (256.f * f8f * f8f * a.wscale);
     const float ms = mm * weight_scale_factor<WFN>() * dinv;
-    constexpr bool kPacked = std::is_same<TS, _Float16>::value && WFN == 0;
+    constexpr bool kPacked = (std::is_same<TS, _Float16>::value || std::is_same<TS, __bf16>::value) && WFN == 0;
     ushort2w wmax2[2] = {{0, 0}, {0, 0}};
     const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f) * dinv);
     const half2w msu = {msu1, msu1};
@@ -1421,7 +1456,7 @@ hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t ele
                                                      inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
                                                       st);
     case FORM_BF16:
-        return launch_mfma_t<R, __bf16, __bf16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
+        return launch_mfma_t<R, __bf16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
                                                      inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
                                                       st);
     default: return hipErrorInvalidValue;
