@@ -9,11 +9,12 @@
 //   A[m][k] = D_s[row m + (dy,dz)][x0 - 8 + k]    (16-bit D plane in LDS)
 //   B[k][n] = W(x0 - 8 + k - (x0 + n), dy, dz)     (band of the row's weights)
 //   C[m][n] += A . B                               (FP32 accumulate)
-// D is a sum of non-negative terms (no cancellation), so 16-bit D planes keep
-// the distance to ~1e-3 relative (BF16) or ~1e-4 (FP16).  The weight, the
-// weighted sums and wmax stay FP32; the final normalisation is FP64.  Volumes
-// are stored normalised, v_s = (v - shift) * scale, which leaves every
-// weight unchanged once the per-slice multiplier is divided by scale^2.
+// D is a sum of non-negative terms (no cancellation), so FP16 D planes keep
+// the distance to ~1e-4 relative (the BF16 storage form widens its values
+// to FP16 before the difference; the FP16_FP8 form scales FP8 planes per
+// tile).  The weighted sums stay FP32 and the final normalisation is FP64.
+// Volumes are stored normalised, v_s = (v - shift) * scale, which leaves
+// every weight unchanged once the per-slice multiplier is divided by scale^2.
 //
 // A workgroup (8 wavefronts) owns a 32x8x8 output tile and stages D_s over
 // the tile plus the patch halo in LDS (double buffered, one barrier per
@@ -55,7 +56,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned char f8_t;
 // FP8-plane plans: at most this many MFMAs per offset and band operands
 constexpr int kF8MaxM = 8, kF8MaxSig = 8;
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Patch rows of the isotropic ellipsoid of radius R and their weight
@@ -305,16 +305,6 @@ struct DPlane<_Float16> {
     static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    }
-};
-template <>
-struct DPlane<__bf16> {
-    static constexpr int XWE = XW, RYPE = RYP;
-    typedef bf16x8 vec8;
-    static __device__ __forceinline__ __bf16 from(float v) { return (__bf16)v; }
-    static __device__ __forceinline__ floatx4 mfma(vec8 a, vec8 b, floatx4 c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 // FP8 e4m3 plane: 32 bytes per lane and operand, v_mfma_f32_16x16x128_f8f6f4
