@@ -782,12 +782,11 @@ void filter_mfma(MfmaArgs<TS> a)
     typedef typename ST4::chunk chunk4;
     constexpr int S8 = T::S8, S4 = T::S4;
     // the chunks at the tile's own positions stay fixed over the offsets.
-    // The FP16 and FP8-plane forms with isotropic patches up to radius 2 keep
-    // them in registers (their LDS reads sat on each step's critical path: 5%
-    // of the FP8-plane pass); the other instances, which would spill in the
-    // loop, keep them in LDS, one slot per (thread, chunk)
-    constexpr bool kP0Reg = WFN == 0 && ISO && R <= 2 &&
-                            (kF8 ? (!PRUNED || R < 2) : (std::is_same<TS, _Float16>::value && std::is_same<TD, _Float16>::value));
+    // Instances with isotropic patches up to radius 2 keep them in registers
+    // (their LDS reads sat on each step's critical path: 5% of the FP8-plane
+    // pass, 3% of the BF16 and FP32_FAST passes); the other instances, which
+    // would spill in the loop, keep them in LDS, one slot per (thread, chunk)
+    constexpr bool kP0Reg = WFN == 0 && ISO && R <= 2 && (kF8 ? (!PRUNED || R < 2) : std::is_same<TD, _Float16>::value);
     constexpr int P0L8 = kP0Reg ? 1 : (S8 > 0 ? S8 * NTM : 1), P0L4 = kP0Reg ? 1 : (S4 > 0 ? S4 * NTM : 1);
     __shared__ __attribute__((aligned(16))) chunk8 p0s8[P0L8];
     __shared__ __attribute__((aligned(16))) chunk4 p0s4[P0L4];
