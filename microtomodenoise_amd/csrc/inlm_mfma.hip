@@ -413,42 +413,36 @@ constexpr int kWeightValu = 13;
 // and the conversion for FP8 planes)
 template <class TS, class TD>
 constexpr int kStageValu = std::is_same<TD, f8_t>::value ? 6 : std::is_same<TS, __bf16>::value ? 12 : sizeof(TS) == 2 ? 6 : 12;
-// per pair of outputs in the packed FP16 form (conversion, scale, polynomial,
-// step, the sums), degree-7 polynomial; degree 5
-constexpr int kWeightValuPacked = 15;
-constexpr int kWeightValuPacked5 = 13;
+// per pair of outputs in the packed FP16 weight (conversion, scale,
+// polynomial, step, the sums)
+constexpr int kWeightValuPacked = 13;
 constexpr float kPadePoly[8] = {0.1679810881614685f,  0.30033349990844727f,  0.2661350965499878f,
                                 0.1601015031337738f,  0.06991086900234222f,  0.024921204894781113f,
                                 0.008454217575490475f, 0.002158894669264555f};
-// FP16 storage form: the weights of two outputs per instruction in packed
+// 16-bit storage forms: the weights of two outputs per instruction in packed
 // FP16.  u = a / (2h) + 1 = d m / 3.56648 + 1, clamped to [0, 1] (u = 0 at
 // and beyond the cutoff).  The weight is computed divided by its value at
-// the cutoff, c0 = 0.025 (the sums and the maximum weight all scale alike,
-// so the result is unchanged): w / c0 = Q(u) u + s(u), with Q a polynomial
-// with FP16 coefficients and s a step that is 1 for u >= 2^-14 and 0 at
-// u = 0.  Dividing by c0 turns the step's product into the last Horner
-// addend, one instruction fewer per pair.  The coefficients minimise the
-// largest deviation of the FP16 evaluation over every FP16 u in [2^-13, 1].
-// Degree 7 (Q of degree 6): on a uniform grid of a (a > -0.999 h) at most
-// 1.6e-3 of the weight from the rational form (2.5e-4 rms), most of it from
-// rounding u to FP16.
-constexpr _Float16 kPadePolyQ[7] = {(_Float16)4.4921875f,   (_Float16)7.85546875f, (_Float16)0.77392578125f,
-                                    (_Float16)28.015625f,   (_Float16)-18.640625f, (_Float16)14.3125f,
-                                    (_Float16)2.189453125f};
-// Degree 5 (Q of degree 4), for the FP8-plane form whose FP8 distances and
-// weights already carry errors near 1e-2: at most 1.7e-3 (3.3e-4 rms), with
-// c0 = 0.0248, two instructions fewer per pair.
+// the cutoff, c0 = 0.0248 (the sums and the maximum weight all scale alike,
+// so the result is unchanged): w / c0 = Q(u) u + s(u), with Q a degree-4
+// polynomial with FP16 coefficients and s a step that is 1 for u >= 2^-14
+// and 0 at u = 0.  Dividing by c0 turns the step's product into the last
+// Horner addend, one instruction fewer per pair.  The coefficients minimise
+// the largest deviation of the FP16 evaluation over every FP16 u in
+// [2^-13, 1]: on a uniform grid of a (a > -0.999 h) the weight deviates by at
+// most 1.7e-3 from the rational form (3.3e-4 rms), most of it from rounding
+// u to FP16.  A degree-7 weight (Q of degree 6) was within 1.6e-3 (2.5e-4
+// rms) and cost two instructions more per pair, for no measurable change in
+// PSNR.
 constexpr _Float16 kPadePolyQ5[5] = {(_Float16)5.2734375f, (_Float16)-0.86474609375f, (_Float16)37.53125f,
                                      (_Float16)-40.03125f, (_Float16)37.40625f};
 typedef _Float16 half2w __attribute__((ext_vector_type(2)));
 typedef unsigned short ushort2w __attribute__((ext_vector_type(2)));
 // the weights (divided by c0) of two pairs of outputs, the two polynomial
-// chains in lockstep (DEG 7: kPadePolyQ, DEG 5: kPadePolyQ5)
-template <int DEG>
+// chains in lockstep
 __device__ __forceinline__ void packed_weights(half2w d0, half2w d1, half2w ms, half2w &w0, half2w &w1)
 {
-    static_assert(DEG == 7 || DEG == 5, "polynomial degree");
-    constexpr const _Float16 *Q = DEG == 7 ? kPadePolyQ : kPadePolyQ5;
+    constexpr int DEG = 5;
+    constexpr const _Float16 *Q = kPadePolyQ5;
     const half2w zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
     half2w u0 = __builtin_elementwise_fma(d0, ms, one), u1 = __builtin_elementwise_fma(d1, ms, one);
     u0 = __builtin_elementwise_min(__builtin_elementwise_max(u0, zero), one);
@@ -969,7 +963,7 @@ void filter_mfma(MfmaArgs<TS> a)
     auto accumulate = [&](const floatx4 &d, const RVals &rv) __attribute__((always_inline)) {
         if constexpr (kPacked) {
             half2w w[2];
-            packed_weights<kF8 ? 5 : 7>(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
+            packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
                            __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -1119,7 +1113,7 @@ void filter_mfma(MfmaArgs<TS> a)
             // barrier) and the weights of the previous offset; then the first
             // MFMA.  With three MFMAs per offset there is little else to cover
             // the LDS latency.
-            constexpr int NVA = kPacked ? 2 * kWeightValuPacked5 : 4 * kWeightValu;
+            constexpr int NVA = kPacked ? 2 * kWeightValuPacked : 4 * kWeightValu;
             constexpr int NVS = kStageValu<TS, TD> * (2 * T::S8 + T::S4);
             __builtin_amdgcn_sched_group_barrier(0x0100, 2 * KD, 0);
             if constexpr (kP0Reg) {
