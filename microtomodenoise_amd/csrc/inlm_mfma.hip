@@ -352,6 +352,7 @@ struct MfmaArgs {
     unsigned char f8tmpl[kF8MaxSig][4][48];
     int f8offa[kF8MaxM][4], f8offb[kF8MaxM][4];
     float wscale;
+    float wmin;  // the smallest patch weight (scaled 16-bit planes)
     // 16-bit storage: R and P are each followed, `shifted` elements further
     // on, by a copy shifted by one element (copy[j] = volume[j + 1])
     long shifted;
@@ -583,6 +584,25 @@ struct Stager<W, _Float16, _Float16> {
         }
         store_dplane<W>(dst, out);
     }
+    // scaled plane D * f^2 clamped to [0, 1]: one FMA (b = -P(y) f) and one
+    // multiply with the instruction's clamp per pair
+    half2v f;
+    __device__ __forceinline__ void own_scaled(chunk &b) const
+    {
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) b[h] = -(b[h] * f);
+    }
+    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    {
+        const half2v zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
+        half2v out[W / 2];
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) {
+            const half2v d = __builtin_elementwise_fma(pn[h], f, b[h]);
+            out[h] = __builtin_elementwise_min(__builtin_elementwise_max(d * d, zero), one);
+        }
+        store_dplane<W>(dst, out);
+    }
 };
 
 // BF16 storage and FP16 plane: each pair of stored values is widened to
@@ -617,6 +637,23 @@ struct Stager<W, __bf16, _Float16> {
     {
 #pragma unroll
         for (int h = 0; h < W / 2; h++) b[h] = __builtin_bit_cast(unsigned, widen(b[h]));
+    }
+    half2v f;
+    __device__ __forceinline__ void own_scaled(chunk &b) const
+    {
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) b[h] = __builtin_bit_cast(unsigned, -(widen(b[h]) * f));
+    }
+    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    {
+        const half2v zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
+        half2v out[W / 2];
+#pragma unroll
+        for (int h = 0; h < W / 2; h++) {
+            const half2v d = __builtin_elementwise_fma(widen(pn[h]), f, __builtin_bit_cast(half2v, b[h]));
+            out[h] = __builtin_elementwise_min(__builtin_elementwise_max(d * d, zero), one);
+        }
+        store_dplane<W>(dst, out);
     }
 };
 
@@ -765,6 +802,25 @@ void filter_mfma(MfmaArgs<TS> a)
         f8f = __builtin_bit_cast(float, (unsigned)(127 + m) << 23);
     }
 
+    // Scaled FP16 planes (FP16 and BF16 storage, Pade weights): the plane
+    // holds D * f^2, f = 2^m per tile, clamped to [0, 1] by the squaring
+    // instruction.  f^2 <= wmin / (4 * 3.56648 / |mm|) with |mm| the smallest
+    // of the tile's slices: a clamped tap alone then adds at least four
+    // times the largest cutoff distance of the tile, so every patch with a
+    // clamped tap stays beyond the cutoff and its weight stays exactly zero.
+    constexpr bool kSC = WFN == 0 && std::is_same<TD, _Float16>::value && sizeof(TS) == 2;
+    float pf = 1.f;
+    if constexpr (kSC) {
+        float smax = 0.f;
+#pragma unroll
+        for (int zz = 0; zz < MTZ; zz++) smax = __builtin_fmaxf(smax, a.sigma[Z0 + zz < g.nz ? Z0 + zz : g.nz - 1]);
+        const float x = __builtin_fabsf((-1.f / (smax * smax * a.beta)) * a.mscale) * a.wmin * (1.f / (4.f * 3.56648f));
+        int e = (int)((__builtin_bit_cast(unsigned, x) >> 23) & 0xffu) - 127;
+        e = e < -24 ? -24 : e > 16 ? 16 : e;
+        const int m = e >> 1;
+        pf = __builtin_bit_cast(float, (unsigned)(127 + m) << 23);
+    }
+
     // Staging chunks: 8 or 4 columns of one region row.  Region row (rz, ry)
     // is slice Z0-R+rz, row Y0-R+ry; LDS column c holds x = X0 - 8 + c
     // (16-bit planes) or x = X0 - 4 + c (FP8 planes).
@@ -788,8 +844,8 @@ void filter_mfma(MfmaArgs<TS> a)
     chunk4 p0r4[kP0Reg && S4 > 0 ? S4 : 1];
     ST8 stg8;
     ST4 stg4;
-    if constexpr (kF8) {
-        const _Float16 fh = (_Float16)f8f;
+    if constexpr (kF8 || kSC) {
+        const _Float16 fh = (_Float16)(kF8 ? f8f : pf);
         stg8.f = half2v{fh, fh};
         stg4.f = half2v{fh, fh};
     }
@@ -815,7 +871,10 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk8 b;
         stg8.load(b, Pt + coff8[c]);
-        if constexpr (kF8 || std::is_same<TS, __bf16>::value) stg8.own(b);
+        if constexpr (kSC)
+            stg8.own_scaled(b);
+        else if constexpr (kF8 || std::is_same<TS, __bf16>::value)
+            stg8.own(b);
         if constexpr (kP0Reg)
             __builtin_memcpy(&p0r8[c], b, sizeof(b));
         else
@@ -832,7 +891,10 @@ void filter_mfma(MfmaArgs<TS> a)
         }
         chunk4 b;
         stg4.load(b, Pt + coff4[c]);
-        if constexpr (kF8 || std::is_same<TS, __bf16>::value) stg4.own(b);
+        if constexpr (kSC)
+            stg4.own_scaled(b);
+        else if constexpr (kF8 || std::is_same<TS, __bf16>::value)
+            stg4.own(b);
         if constexpr (kP0Reg)
             __builtin_memcpy(&p0r4[c], b, sizeof(b));
         else
@@ -884,6 +946,7 @@ void filter_mfma(MfmaArgs<TS> a)
     unsigned wmax_bits[4] = {0u, 0u, 0u, 0u};
     float dinv = 1.f;  // FP8 planes: 1 / (S * wscale)
     if constexpr (kF8) dinv = 1.f / (256.f * f8f * f8f * a.wscale);
+    if constexpr (kSC) dinv = 1.f / (pf * pf);
     const float ms = mm * weight_scale_factor<WFN>() * dinv;
     constexpr bool kPacked = (std::is_same<TS, _Float16>::value || std::is_same<TS, __bf16>::value) && WFN == 0;
     ushort2w wmax2[2] = {{0, 0}, {0, 0}};
@@ -939,9 +1002,19 @@ void filter_mfma(MfmaArgs<TS> a)
     };
     auto stage = [&](TD *buf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < S8; c++) stg8.store(pn8[c], kP0Reg ? p0r8[c] : p0s8[t + c * NTM], buf + loff8[c]);
+        for (int c = 0; c < S8; c++) {
+            if constexpr (kSC)
+                stg8.store_scaled(pn8[c], kP0Reg ? p0r8[c] : p0s8[t + c * NTM], buf + loff8[c]);
+            else
+                stg8.store(pn8[c], kP0Reg ? p0r8[c] : p0s8[t + c * NTM], buf + loff8[c]);
+        }
 #pragma unroll
-        for (int c = 0; c < S4; c++) stg4.store(pn4[c], kP0Reg ? p0r4[c] : p0s4[t + c * NTM], buf + loff4[c]);
+        for (int c = 0; c < S4; c++) {
+            if constexpr (kSC)
+                stg4.store_scaled(pn4[c], kP0Reg ? p0r4[c] : p0s4[t + c * NTM], buf + loff4[c]);
+            else
+                stg4.store(pn4[c], kP0Reg ? p0r4[c] : p0s4[t + c * NTM], buf + loff4[c]);
+        }
     };
     // R at this lane's four outputs shifted by one search offset, in the
     // stored type
@@ -1392,6 +1465,10 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
     a.ntz = (g.nz + MTZ - 1) / MTZ;
     if (!mfma_bands_r<R>(taps.dxyz, taps.w, taps.n, a.band)) return hipErrorInvalidValue;
     a.wscale = 1.f;
+    a.wmin = 1.f;
+    for (int p = 0; p < MRowsOf<R>::value.npat; p++)
+        for (int dx = -MRowsOf<R>::value.pj[p]; dx <= MRowsOf<R>::value.pj[p]; dx++)
+            if (a.band[p][dx + R] > 0.f) a.wmin = std::min(a.wmin, a.band[p][dx + R]);
     std::memset(a.f8tmpl, 0, sizeof(a.f8tmpl));
     std::memset(a.f8offa, 0, sizeof(a.f8offa));
     std::memset(a.f8offb, 0, sizeof(a.f8offb));
