@@ -6,7 +6,7 @@
  * names, nesting and defaults, so code written against the reference compiles
  * unchanged against this build.  Three fields are this build's own and sit in
  * the GPU block: the arithmetic form, the weight function and the sample
- * seed of the noise estimator.
+ * seed of the noise estimator, and a device memory limit.
  */
 #ifndef INLM_DENOISE_PARAMETERS_H
 #define INLM_DENOISE_PARAMETERS_H
@@ -28,7 +28,7 @@ struct DenoiseParameters {
     float z_anisotropy = 1.f;
 
     struct NoiseLevel {
-        std::string mode = "z-adaptive";  // "z-adaptive", "global" ("auto"), "manual"
+        std::string mode = "z-adaptive";  // "z-adaptive", "global" ("auto"), "manual", "semimanual"
         int zwindow = 100;
         float sigma[2] = {0.0f, 0.0f};    // manual mode
         uint64_t n_samples = 100000;
@@ -52,6 +52,10 @@ struct DenoiseParameters {
         std::string precision = "fp32";  // "fp32" (parity), "fp32_fast", "bf16", "fp16", "fp16_fp8"
         std::string weight_fn = "pade";  // "pade" (reference CPU) or "exp" (reference GPU)
         uint64_t sample_seed = 0;
+        // device bytes configure_device may plan for (0: the free device
+        // memory); below the volume's footprint it returns fewer slices than
+        // shape[2] and the caller takes the block path
+        int64_t memory_limit = 0;
     };
     struct IO {
         bool resume = false;

@@ -43,8 +43,16 @@ typedef enum {
 } inlm_status;
 
 /* NoiseLevel.mode of denoise_parameters.h:23 ("global"/"auto",
- * "z-adaptive", "manual"). */
-typedef enum { INLM_NOISE_GLOBAL = 0, INLM_NOISE_ZADAPTIVE = 1, INLM_NOISE_MANUAL = 2 } inlm_noise_mode;
+ * "z-adaptive", "manual", "semimanual"; noiselevel.cpp:14-83).  Semimanual:
+ * the first estimate is sigma[0] on every slice and picks the sample whose
+ * patch std lies closest to it; later estimates are that sample's patch std
+ * on the current result (noiselevel.cpp:56-79). */
+typedef enum {
+    INLM_NOISE_GLOBAL = 0,
+    INLM_NOISE_ZADAPTIVE = 1,
+    INLM_NOISE_MANUAL = 2,
+    INLM_NOISE_SEMIMANUAL = 3
+} inlm_noise_mode;
 
 /* Arithmetic form of the filter kernel.
  *   FP32      -- parity form: operation-faithful to the reference CPU path
@@ -98,8 +106,16 @@ typedef struct inlm_params {
     /* build-specific */
     int32_t precision;             /* inlm_precision, FP32 */
     int32_t weight_fn;             /* inlm_weight_fn, PADE */
-    int32_t kernel_select;         /* 0: tiled kernel where one covers the patch, 1: per-voxel kernel */
-    int32_t reserved[7];
+    /* FP32 form: 0 the tiled kernel where one covers the patch, 1 the
+     * per-voxel kernel.  Matrix-core forms: 0 the fastest kernel for the
+     * case (at patch radius 2 with 16-bit storage and FP16 planes, the
+     * 4x4-block kernel), 2 the row kernel. */
+    int32_t kernel_select;
+    /* NoiseLevel.circular_mask_diameter (denoise_parameters.h:33): when > 0
+     * noise samples are drawn only inside that circle around the slice
+     * centre (noiselevel.cpp:204-224) */
+    int32_t circular_mask_diameter; /* 0 */
+    int32_t reserved[6];
 } inlm_params;
 
 typedef struct inlm_handle inlm_handle;
@@ -151,6 +167,15 @@ INLM_API inlm_status inlm_get_result(inlm_handle *h, float *out, int32_t is_devi
 INLM_API inlm_status inlm_draw_samples(const int32_t shape[3], int64_t n, int32_t patchsize, uint64_t seed,
                                        int64_t *out);
 
+/* inlm_draw_samples restricted to a circular field of view
+ * (noiselevel.cpp:204-224): with mask_diameter > 0, a sample's corner (x, y)
+ * is redrawn until (x - nx/2)^2 + (y - ny/2)^2 <= (mask_diameter/2 -
+ * sqrt(2 patchsize^2))^2 (integer square, float threshold, as the
+ * reference).  BAD_ARG when no position of the sampling range satisfies it.
+ * mask_diameter <= 0 is inlm_draw_samples.  Host-only. */
+INLM_API inlm_status inlm_draw_samples_mask(const int32_t shape[3], int64_t n, int32_t patchsize,
+                                            int32_t mask_diameter, uint64_t seed, int64_t *out);
+
 /* Per-sample 15x15 (patchsize^2) in-slice standard deviation on the GPU
  * (NoiseLevel::get_patchstd, noiselevel.cpp:235-283), one wavefront per
  * sample, FP64 accumulation.  which = 0: the original volume; 1: the last
@@ -162,6 +187,18 @@ INLM_API inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t 
  * z-adaptive, sigma = mean + stds_from_mean * std.  Writes nz floats. */
 INLM_API inlm_status inlm_estimate_noise(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n,
                                          float *sigma_out);
+
+/* The host half of get_noiselevel (noiselevel.cpp:9-95) on per-sample patch
+ * standard deviations `stds` (n values for the sorted sample list): FP64
+ * two-cluster EM, global or z-adaptive (with the reference's window and its
+ * extrapolation into slices without valid samples, noiselevel.cpp:529-606),
+ * manual, or semimanual.  which = 0 is the first estimate of a run, 1 a
+ * later one.  Semimanual keeps its reference sample in *semimanual_ref:
+ * which = 0 writes it (the sample closest to sigma[0]), which = 1 reads it.
+ * Writes nz floats.  Host-only. */
+INLM_API inlm_status inlm_sigma_from_patchstd(const inlm_params *p, const int32_t shape[3], const int64_t *samples,
+                                              const double *stds, int64_t n, int32_t which, int64_t *semimanual_ref,
+                                              float *sigma_out);
 
 /* The whole schedule of main.cpp:325-545 on a resident volume: sigma0 from
  * the original, iteration 1, sigma1 re-estimated on its output with the same
@@ -184,6 +221,29 @@ INLM_API inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, 
  * by its own moments. */
 INLM_API inlm_status inlm_denoise_slabs(const inlm_params *p, const int32_t shape[3], int32_t device,
                                         const float *raw, float *out, int32_t slab_slices, float *sigma_log);
+
+/* Capacity of one GPU for the slab drivers, as configure_device sizes it
+ * (iternlm_gpu.cu:1528-1566): the largest number of output slices whose slab
+ * (with the filter's z reach on both sides) a handle of these parameters
+ * holds within `memory_limit` bytes of device memory, or within the
+ * device's free memory when memory_limit <= 0.  nz when the whole volume
+ * fits; OOM when not even one slice does. */
+INLM_API inlm_status inlm_capacity_slices(const inlm_params *p, const int32_t shape[3], int32_t device,
+                                          int64_t memory_limit, int32_t *n_slices);
+
+/* One filter iteration over z-slabs of at most slab_slices output slices
+ * on host buffers: the GPU half of Run_GaussianNoise_GPUBlocks
+ * (iternlm_gpu.cu:1729-1882).  raw: the original volume; prev: the previous
+ * iterate (ignored at iter 1, where the original is the previous iterate);
+ * sigma: nz floats indexed by global z; out: nx*ny*nz floats (may alias
+ * prev: the slabs read their halo from a copy of the previous iterate, not
+ * from slices the same call has already overwritten, unlike the
+ * reference's in-place update).  Every slab is uploaded with the filter's z
+ * reach (search |dz| plus patch |dz|) on both sides, so the FP32 form gives
+ * the whole-volume iteration bit for bit. */
+INLM_API inlm_status inlm_iterate_slabs(const inlm_params *p, const int32_t shape[3], int32_t device, int32_t iter,
+                                        const float *raw, const float *prev, const float *sigma, float *out,
+                                        int32_t slab_slices);
 
 /* Host-only query of the tables a handle with these parameters would use on
  * a volume of `shape`: search offsets and patch taps as (dx,dy,dz) triples in

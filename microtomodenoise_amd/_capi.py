@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libinlm.so")
 
 INLM_OK, INLM_NO_DEVICE, INLM_HIP_ERROR, INLM_BAD_ARG, INLM_OOM, INLM_IO_ERROR = 0, -1, -2, -3, -4, -5
-NOISE_GLOBAL, NOISE_ZADAPTIVE, NOISE_MANUAL = 0, 1, 2
+NOISE_GLOBAL, NOISE_ZADAPTIVE, NOISE_MANUAL, NOISE_SEMIMANUAL = 0, 1, 2, 3
 PREC_FP32, PREC_FP32_FAST, PREC_BF16, PREC_FP16, PREC_FP16_FP8 = 0, 1, 2, 3, 4
 WEIGHT_PADE, WEIGHT_EXP = 0, 1
 
@@ -37,6 +37,8 @@ EXPORTS = [
     "inlm_noise_csv_write", "inlm_noise_csv_read", "inlm_iteration_on_disk", "inlm_default_io_options",
     "inlm_denoise_files", "inlm_denoise_slabs", "inlm_default_rgb_options", "inlm_denoise_rgb",
     "inlm_rgb_tiff_shape", "inlm_rgb_tiff_read", "inlm_rgb_tiff_write", "inlm_denoise_rgb_files",
+    # noise estimator host half, circular sampling mask, slab drivers
+    "inlm_draw_samples_mask", "inlm_sigma_from_patchstd", "inlm_capacity_slices", "inlm_iterate_slabs",
 ]
 
 
@@ -85,7 +87,8 @@ class InlmParams(C.Structure):
         ("precision", C.c_int32),
         ("weight_fn", C.c_int32),
         ("kernel_select", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("circular_mask_diameter", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -167,6 +170,12 @@ def lib():
                                              C.c_char_p, C.POINTER(InlmIoOptions)]
         L.inlm_denoise_slabs.argtypes = [C.POINTER(InlmParams), C.POINTER(C.c_int32), C.c_int32, f32p, f32p,
                                          C.c_int32, f32p]
+        L.inlm_draw_samples_mask.argtypes = [i32p, C.c_int64, C.c_int32, C.c_int32, C.c_uint64, i64p]
+        L.inlm_sigma_from_patchstd.argtypes = [C.POINTER(InlmParams), i32p, i64p, f64p, C.c_int64, C.c_int32, i64p,
+                                               f32p]
+        L.inlm_capacity_slices.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, C.c_int64, i32p]
+        L.inlm_iterate_slabs.argtypes = [C.POINTER(InlmParams), i32p, C.c_int32, C.c_int32, f32p, f32p, f32p, f32p,
+                                         C.c_int32]
         _LIB = L
     return _LIB
 
@@ -193,12 +202,42 @@ def default_params() -> InlmParams:
     return p
 
 
-def draw_samples(shape_xyz, n: int, patchsize: int = 15, seed: int = 0) -> np.ndarray:
-    """Sample positions of the noise estimator (host-only, no GPU needed)."""
+def draw_samples(shape_xyz, n: int, patchsize: int = 15, seed: int = 0, mask_diameter: int = 0) -> np.ndarray:
+    """Sample positions of the noise estimator (host-only, no GPU needed);
+    mask_diameter > 0 keeps them inside the circular field of view."""
     out = np.zeros(n, np.int64)
     sh = np.asarray(shape_xyz, np.int32)
-    check(lib().inlm_draw_samples(ptr(sh, C.c_int32), n, patchsize, seed, ptr(out, C.c_int64)))
+    if mask_diameter > 0:
+        check(lib().inlm_draw_samples_mask(ptr(sh, C.c_int32), n, patchsize, mask_diameter, seed,
+                                           ptr(out, C.c_int64)))
+    else:
+        check(lib().inlm_draw_samples(ptr(sh, C.c_int32), n, patchsize, seed, ptr(out, C.c_int64)))
     return out
+
+
+def sigma_from_patchstd(params: InlmParams, shape_xyz, samples, stds, which: int = 0, semimanual_ref=None):
+    """The host half of the noise estimator (inlm_sigma_from_patchstd):
+    sigma per slice from per-sample patch stds.  semimanual_ref is a
+    one-element int64 array that carries the semimanual reference sample
+    from the first estimate (which=0) to later ones."""
+    sh = np.asarray(shape_xyz, np.int32)
+    smp = np.ascontiguousarray(samples, np.int64)
+    sd = np.ascontiguousarray(stds, np.float64)
+    out = np.zeros(int(sh[2]), np.float32)
+    ref = semimanual_ref if semimanual_ref is not None else np.full(1, -1, np.int64)
+    check(lib().inlm_sigma_from_patchstd(C.byref(params), ptr(sh, C.c_int32), ptr(smp, C.c_int64),
+                                         ptr(sd, C.c_double), len(smp), which, ptr(ref, C.c_int64),
+                                         ptr(out, C.c_float)))
+    return out
+
+
+def capacity_slices(params: InlmParams, shape_xyz, device: int = 0, memory_limit: int = 0) -> int:
+    """Output slices one slab may hold within memory_limit device bytes (the
+    free device memory when 0): the sizing of configure_device."""
+    sh = np.asarray(shape_xyz, np.int32)
+    n = C.c_int32()
+    check(lib().inlm_capacity_slices(C.byref(params), ptr(sh, C.c_int32), device, memory_limit, C.byref(n)))
+    return n.value
 
 
 def tables(params: InlmParams, shape_xyz):

@@ -25,6 +25,65 @@ constexpr size_t kGuard = 64;
 
 namespace inlm {
 void set_global_error(const std::string &msg) { g_create_error = msg; }
+
+inlm_status sigma_from_std(const inlm_params &p, const int shape[3], const int64_t *samples, const double *sd,
+                           int64_t n, int which, int64_t &semi_ref, float *sig, std::string &err)
+{
+    const int nz = shape[2];
+    const double k = p.stds_from_mean;
+    switch (p.noise_mode) {
+    case INLM_NOISE_MANUAL: {
+        // sigma[0] on the first estimate, sigma[1] afterwards (noiselevel.cpp:81-88)
+        const float v = p.sigma[which == 0 ? 0 : 1];
+        for (int i = 0; i < nz; i++) sig[i] = v;
+        return INLM_OK;
+    }
+    case INLM_NOISE_SEMIMANUAL: {
+        // noiselevel.cpp:56-79: the first estimate is sigma[0] and fixes the
+        // sample whose patch std lies closest to it (the first of equals);
+        // later estimates are that sample's patch std
+        if (n < 1) {
+            err = "semimanual noise mode needs noise samples";
+            return INLM_BAD_ARG;
+        }
+        float v;
+        if (which == 0) {
+            double best = 1e9;
+            int64_t ref = 0;
+            for (int64_t i = 0; i < n; i++) {
+                const double d = std::fabs(sd[i] - (double)p.sigma[0]);
+                if (d < best) {
+                    best = d;
+                    ref = i;
+                }
+            }
+            semi_ref = ref;
+            v = p.sigma[0];
+        } else {
+            if (semi_ref < 0 || semi_ref >= n) {
+                err = "semimanual noise mode: no reference sample from a first estimate";
+                return INLM_BAD_ARG;
+            }
+            v = (float)sd[semi_ref];
+        }
+        for (int i = 0; i < nz; i++) sig[i] = v;
+        return INLM_OK;
+    }
+    case INLM_NOISE_ZADAPTIVE: {
+        std::vector<double> mean, stdev;
+        zwindow_clusters(samples, sd, n, shape, p.zwindow, mean, stdev);
+        for (int i = 0; i < nz; i++) sig[i] = (float)(mean[i] + k * stdev[i]);
+        return INLM_OK;
+    }
+    case INLM_NOISE_GLOBAL: {
+        double mean = 0, stdev = 0;
+        create2clusters(sd, (int)n, 0, (int)n - 1, mean, stdev);
+        for (int i = 0; i < nz; i++) sig[i] = (float)(mean + k * stdev);
+        return INLM_OK;
+    }
+    default: err = "unknown noise mode"; return INLM_BAD_ARG;
+    }
+}
 }  // namespace inlm
 
 struct inlm_handle {
@@ -58,6 +117,7 @@ struct inlm_handle {
     int64_t *samples_dev = nullptr;
     double *std_dev = nullptr;
     int64_t samples_cap = 0;
+    int64_t semi_ref = -1;  // semimanual noise mode: the reference sample of the first estimate
     bool raw_loaded = false, sigma_set = false;
     int last_iter = 0;
     float last_ms = 0.f;
@@ -167,6 +227,35 @@ inlm_status ensure_samples(inlm_handle *h, int64_t n)
 }
 
 }  // namespace
+
+namespace inlm {
+// Device bytes a handle of these parameters allocates for `shape` (the sums
+// of inlm_create's allocations, plus the sample buffers of one noise
+// estimate with n_samples samples).
+size_t handle_bytes(const inlm_params &pin, const int32_t shape_in[3])
+{
+    inlm_params p = pin;
+    int shape[3] = {shape_in[0], shape_in[1], shape_in[2]};
+    if (shape[2] == 1) {
+        p.radius_searchspace[2] = 0;
+        p.radius_patchspace[2] = 0;
+    }
+    Geometry g = make_geometry(shape, p.radius_searchspace, p.radius_patchspace, p.nslices);
+    const size_t esz = form_elem_size(p.precision);
+    if (esz == 2 && (g.nxp & 1) != 0) g.nxp += 1;
+    const size_t pad_elems = padded_alloc_elems(g);
+    const bool fast = p.precision != INLM_PREC_FP32;
+    const size_t shifted = fast && esz == 2 ? (pad_elems + kGuard + 63) / 64 * 64 : 0;
+    const size_t padb = (shifted > 0 ? kGuard + shifted + pad_elems : pad_elems + kGuard) * esz;
+    const size_t vox = (size_t)g.voxels();
+    size_t b = 2 * padb + sizeof(float) * vox + sizeof(float) * (size_t)g.nz;
+    if (fast) b += sizeof(float) * vox + 2 * sizeof(double);
+    const size_t ns = search_offsets(p.radius_searchspace, p.nslices).size();
+    b += 2 * ns * sizeof(SearchEntry);
+    if (p.noise_mode != INLM_NOISE_MANUAL && p.n_samples > 0) b += (size_t)p.n_samples * (sizeof(int64_t) + sizeof(double));
+    return b;
+}
+}  // namespace inlm
 
 extern "C" {
 
@@ -473,19 +562,25 @@ inlm_status run_iteration_impl(inlm_handle *h, int32_t iter, const float *raw, f
     HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
     if (h->fast) {
         inlm::OffsetLists ol{h->lists_dev, h->counts_dev, h->lists_K, nt};
-        const bool pruned = h->probs_dev || h->lists_fixed;
+        bool pruned = h->probs_dev || h->lists_fixed;
         if (h->probs_dev) {
             // fresh lists for this iteration: top-K, fallback, audit draw
             HIP_TRY(h, hipMemsetAsync(h->stats_dev, 0, sizeof(float) * 4, h->stream));
             HIP_TRY(h, inlm::launch_topk(h->probs_dev, (int)nt, ns, h->prop_K, h->prop_thr, h->prop_audit,
                                          h->prop_seed + 0x9E3779B97F4A7C15ull * (uint64_t)iter, h->lists_dev,
                                          h->counts_dev, h->stats_dev, h->stream));
+            // when every tile falls back or is audited, the full-search
+            // kernel runs instead of the pruned one
+            float full_tiles = 0.f;
+            HIP_TRY(h, hipMemcpyAsync(&full_tiles, h->stats_dev, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(h, hipStreamSynchronize(h->stream));
+            if (full_tiles >= (float)nt) pruned = false;
         }
         HIP_TRY(h, inlm::launch_filter_mfma(h->p.precision, h->rp_tiled, h->p.weight_fn, h->Rs, P, h->pad_elems, h->shifted,
                                             h->raw, h->result, h->sigma, h->p.beta,
                                             (float)(1.0 / (h->scale * h->scale)), h->shift, 1.0 / h->scale,
                                             h->search_dev, h->search_fast_dev, ns, h->min_lin, h->taps,
-                                            h->fg, h->stream, pruned ? &ol : nullptr));
+                                            h->fg, h->stream, pruned ? &ol : nullptr, h->p.kernel_select == 2));
     } else {
         HIP_TRY(h, inlm::launch_filter(h->p.precision, h->rp_tiled, h->p.weight_fn, h->R_pad,
                                        static_cast<const float *>(P), h->result, h->sigma, h->p.beta, h->search_dev,
@@ -535,6 +630,41 @@ inlm_status inlm_draw_samples(const int32_t shape[3], int64_t n, int32_t patchsi
     return INLM_OK;
 }
 
+inlm_status inlm_draw_samples_mask(const int32_t shape[3], int64_t n, int32_t patchsize, int32_t mask_diameter,
+                                   uint64_t seed, int64_t *out)
+{
+    if (!shape || (!out && n > 0)) return fail(nullptr, INLM_BAD_ARG, "null argument");
+    int sh[3] = {shape[0], shape[1], shape[2]};
+    if (!inlm::draw_samples(sh, n, patchsize, seed, out, mask_diameter))
+        return fail(nullptr, INLM_BAD_ARG,
+                    mask_diameter > 0 ? "no sample position lies inside the circular mask (or the volume is smaller "
+                                        "than the noise patch)"
+                                      : "volume smaller than the noise patch");
+    return INLM_OK;
+}
+
+inlm_status inlm_sigma_from_patchstd(const inlm_params *p, const int32_t shape[3], const int64_t *samples,
+                                     const double *stds, int64_t n, int32_t which, int64_t *semimanual_ref,
+                                     float *sigma_out)
+{
+    if (!p || !shape || !sigma_out || (n > 0 && (!samples || !stds)) || n < 0)
+        return fail(nullptr, INLM_BAD_ARG, "null argument");
+    if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1) return fail(nullptr, INLM_BAD_ARG, "shape must be positive");
+    if (p->noise_mode == INLM_NOISE_SEMIMANUAL && !semimanual_ref)
+        return fail(nullptr, INLM_BAD_ARG, "semimanual noise mode needs semimanual_ref");
+    const int64_t nslice = (int64_t)shape[0] * shape[1];
+    for (int64_t i = 0; i < n; i++)
+        if (samples[i] < 0 || samples[i] >= nslice * shape[2] || (i > 0 && samples[i] < samples[i - 1]))
+            return fail(nullptr, INLM_BAD_ARG, "samples must be sorted voxel indices of the volume");
+    int sh[3] = {shape[0], shape[1], shape[2]};
+    int64_t dummy = -1;
+    std::string err;
+    const inlm_status s = inlm::sigma_from_std(*p, sh, samples, stds, n, which,
+                                               semimanual_ref ? *semimanual_ref : dummy, sigma_out, err);
+    if (s != INLM_OK) return fail(nullptr, s, err);
+    return INLM_OK;
+}
+
 inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, double *out)
 {
     if (!h || (!samples && n > 0) || (!out && n > 0)) return fail(h, INLM_BAD_ARG, "null argument");
@@ -581,28 +711,15 @@ inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples,
 inlm_status inlm_estimate_noise(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, float *sigma_out)
 {
     if (!h || !sigma_out) return fail(h, INLM_BAD_ARG, "null argument");
-    const int nz = h->geo.nz;
-    if (h->p.noise_mode == INLM_NOISE_MANUAL) {
-        // manual mode: sigma[0] on the first estimate, sigma[1] afterwards
-        // (noiselevel.cpp:81-88).
-        const float s = h->p.sigma[which == 0 ? 0 : 1];
-        for (int i = 0; i < nz; i++) sigma_out[i] = s;
-        return INLM_OK;
-    }
     std::vector<double> sd((size_t)std::max<int64_t>(n, 1));
-    inlm_status s = inlm_patchstd(h, which, samples, n, sd.data());
-    if (s != INLM_OK) return s;
-    const double k = h->p.stds_from_mean;
-    if (h->p.noise_mode == INLM_NOISE_ZADAPTIVE) {
-        std::vector<double> mean, stdev;
-        int shape[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
-        inlm::zwindow_clusters(samples, sd.data(), n, shape, h->p.zwindow, mean, stdev);
-        for (int i = 0; i < nz; i++) sigma_out[i] = (float)(mean[i] + k * stdev[i]);
-    } else {
-        double mean = 0, stdev = 0;
-        inlm::create2clusters(sd.data(), (int)n, 0, (int)n - 1, mean, stdev);
-        for (int i = 0; i < nz; i++) sigma_out[i] = (float)(mean + k * stdev);
+    if (h->p.noise_mode != INLM_NOISE_MANUAL) {
+        inlm_status s = inlm_patchstd(h, which, samples, n, sd.data());
+        if (s != INLM_OK) return s;
     }
+    int shape[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
+    std::string err;
+    const inlm_status s = inlm::sigma_from_std(h->p, shape, samples, sd.data(), n, which, h->semi_ref, sigma_out, err);
+    if (s != INLM_OK) return fail(h, s, err);
     return INLM_OK;
 }
 
@@ -617,8 +734,9 @@ inlm_status inlm_denoise(inlm_handle *h, const float *raw, float *out, float *si
     if (h->p.noise_mode != INLM_NOISE_MANUAL) {
         samples.resize((size_t)h->p.n_samples);
         int sh[3] = {h->geo.nx, h->geo.ny, h->geo.nz};
-        if (!inlm::draw_samples(sh, h->p.n_samples, h->p.patchsize, h->p.sample_seed, samples.data()))
-            return fail(h, INLM_BAD_ARG, "volume smaller than the noise patch");
+        if (!inlm::draw_samples(sh, h->p.n_samples, h->p.patchsize, h->p.sample_seed, samples.data(),
+                                h->p.circular_mask_diameter))
+            return fail(h, INLM_BAD_ARG, "no noise sample fits the volume (or the circular mask)");
     }
     const int64_t ns = (int64_t)samples.size();
     std::vector<float> s0(nz), s1(nz), tmp(nz);

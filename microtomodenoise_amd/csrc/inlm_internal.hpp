@@ -1,6 +1,7 @@
 // Library-internal helpers shared between translation units.
 #pragma once
 #include <cstddef>
+#include <cstdint>
 #include <functional>
 #include <string>
 
@@ -10,6 +11,16 @@ namespace inlm {
 
 // message returned by inlm_last_error(NULL)
 void set_global_error(const std::string &msg);
+
+// The host half of get_noiselevel (noiselevel.cpp:9-95): sigma per slice
+// from the per-sample patch stds sd[0..n) of the sorted sample list, in the
+// parameters' noise mode.  which = 0: first estimate; semi_ref keeps the
+// semimanual reference sample between estimates.
+inlm_status sigma_from_std(const inlm_params &p, const int shape[3], const int64_t *samples, const double *sd,
+                           int64_t n, int which, int64_t &semi_ref, float *sig, std::string &err);
+
+// Device bytes a handle of these parameters allocates for `shape`.
+size_t handle_bytes(const inlm_params &p, const int32_t shape[3]);
 
 // Anscombe transform and its algebraic inverse (Color/colorspace.h:87-123)
 void anscombe(float *v, size_t n);

@@ -786,7 +786,8 @@ inlm_status inlm_denoise_files(const inlm_params *pin, int32_t device, const cha
     std::vector<int64_t> samples;
     if (p.noise_mode != INLM_NOISE_MANUAL) {
         samples.resize((size_t)p.n_samples);
-        if ((s = inlm_draw_samples(shape, p.n_samples, p.patchsize, p.sample_seed, samples.data())) != INLM_OK)
+        if ((s = inlm_draw_samples_mask(shape, p.n_samples, p.patchsize, p.circular_mask_diameter, p.sample_seed,
+                                        samples.data())) != INLM_OK)
             return bail(s);
     }
     const int64_t nsmp = (int64_t)samples.size();
@@ -864,9 +865,10 @@ inlm_status inlm_denoise_files(const inlm_params *pin, int32_t device, const cha
         lf << ctime(&now);
         lf << "ran iterNLM (MI355X) for " << inpath << "\n";
         lf << "-------------------------------------------------------------------------\n";
-        const char *mode = p.noise_mode == INLM_NOISE_ZADAPTIVE ? "z-adaptive"
-                           : p.noise_mode == INLM_NOISE_GLOBAL  ? "global"
-                                                                : "manual";
+        const char *mode = p.noise_mode == INLM_NOISE_ZADAPTIVE   ? "z-adaptive"
+                           : p.noise_mode == INLM_NOISE_GLOBAL    ? "global"
+                           : p.noise_mode == INLM_NOISE_SEMIMANUAL ? "semimanual"
+                                                                  : "manual";
         if (p.noise_mode == INLM_NOISE_ZADAPTIVE) {
             lf << "    noise estimate: z-adaptive (moving window " << p.zwindow << ")\n";
             lf << "    sigma0(slice " << nz / 2 << "): " << s0[nz / 2] << "\n";

@@ -92,7 +92,8 @@ hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, c
                               long shifted, const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search,
                               const SearchEntry *search_full, int n_search, int min_lin, const TapTable &taps,
-                              const FilterGeom &g, hipStream_t st, const OffsetLists *ol = nullptr);
+                              const FilterGeom &g, hipStream_t st, const OffsetLists *ol = nullptr,
+                              bool row_form = false);
 hipError_t launch_pad_normalise(int form, const float *in, void *out, long shifted, const FilterGeom &g, float shift,
                                 float scale, hipStream_t st);
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);

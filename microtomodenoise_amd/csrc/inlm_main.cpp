@@ -26,8 +26,9 @@ const char *kUsage =
     "            --rgb/--color (2D RGB images)  --independent  --nopoisson  --noaverage\n"
     "  denoise:  -maxiter/-iter/-iterations N  -slices/-nslices N  -a/-alpha F\n"
     "            -search N  -search0/1/2 N  -patch N  -patch0/1/2 N\n"
-    "  noise:    -noise/-noisemode global|z-adaptive|manual  -sigma/-s F  -sigma0/-s0 F  -sigma1/-s1 F\n"
-    "            -nsamples/-noise_samples N  -noisepatch N  -noiseshift F  --continuous  --preserve_detail\n"
+    "  noise:    -noise/-noisemode global|auto|z-adaptive|manual|semimanual  -sigma/-s F  -sigma0/-s0 F  -sigma1/-s1 F\n"
+    "            -nsamples/-noise_samples N  -noisepatch N  -noiseshift F  -mask/-mask_diameter N\n"
+    "            --continuous  --preserve_detail\n"
     "  hardware: -gpu0/-device N  -gpus N (one GPU is used)  --blocks  --noblocks  --verbose/--v\n"
     "  this build: -precision fp32|fp32_fast|fp16|bf16|fp16_fp8  -weight pade|exp  -seed N  --print-params\n";
 
@@ -41,7 +42,10 @@ int fail(const char *what, const char *arg)
 
 const char *noise_name(int m)
 {
-    return m == INLM_NOISE_GLOBAL ? "global" : m == INLM_NOISE_MANUAL ? "manual" : "z-adaptive";
+    return m == INLM_NOISE_GLOBAL       ? "global"
+           : m == INLM_NOISE_MANUAL     ? "manual"
+           : m == INLM_NOISE_SEMIMANUAL ? "semimanual"
+                                        : "z-adaptive";
 }
 const char *precision_name(int p)
 {
@@ -128,10 +132,12 @@ int main(int argc, char **argv)
         // noise estimation
         else if (is(a, "-noise") || is(a, "-noisemode")) {
             NEED(a);
-            if (is(v, "global")) p.noise_mode = INLM_NOISE_GLOBAL;
+            if (is(v, "global") || is(v, "auto")) p.noise_mode = INLM_NOISE_GLOBAL;
             else if (is(v, "z-adaptive")) p.noise_mode = INLM_NOISE_ZADAPTIVE;
             else if (is(v, "manual")) p.noise_mode = INLM_NOISE_MANUAL;
-            else return fail("noise mode not supported by this build (global, z-adaptive, manual)", v);
+            else if (is(v, "semimanual")) p.noise_mode = INLM_NOISE_SEMIMANUAL;
+            // the reference prints an error and leaves sigma at zero (noiselevel.cpp:89-90)
+            else return fail("unknown noise estimation mode", v);
         }
         else if (is(a, "-sigma") || is(a, "-s")) { NEED(a); p.sigma[0] = p.sigma[1] = (float)std::atof(v); }
         else if (is(a, "-sigma0") || is(a, "-s0")) { NEED(a); p.sigma[0] = (float)std::atof(v); }
@@ -139,10 +145,7 @@ int main(int argc, char **argv)
         else if (is(a, "-nsamples") || is(a, "-noise_samples")) { NEED(a); p.n_samples = std::atoll(v); }
         else if (is(a, "-noisepatch")) { NEED(a); p.patchsize = std::atoi(v); }
         else if (is(a, "-noiseshift")) { NEED(a); p.stds_from_mean = (float)std::atof(v); }
-        else if (is(a, "-mask") || is(a, "-mask_diameter")) {
-            NEED(a);
-            if (std::atoi(v) > 0) return fail("the circular sampling mask is not supported by this build", v);
-        }
+        else if (is(a, "-mask") || is(a, "-mask_diameter")) { NEED(a); p.circular_mask_diameter = std::atoi(v); }
         else if (is(a, "--continuous")) p.continuous_estimate = 1;
         else if (is(a, "--preserve_detail")) {
             // the reference's suggested setting (main.cpp:195-202)
@@ -185,9 +188,10 @@ int main(int argc, char **argv)
         std::printf("search=%d,%d,%d\npatch=%d,%d,%d\n", p.radius_searchspace[0], p.radius_searchspace[1],
                     p.radius_searchspace[2], p.radius_patchspace[0], p.radius_patchspace[1],
                     p.radius_patchspace[2]);
-        std::printf("noise=%s\nsigma=%g,%g\nn_samples=%lld\npatchsize=%d\nstds_from_mean=%g\ncontinuous=%d\n",
+        std::printf("noise=%s\nsigma=%g,%g\nn_samples=%lld\npatchsize=%d\nstds_from_mean=%g\ncontinuous=%d\n"
+                    "mask_diameter=%d\n",
                     noise_name(p.noise_mode), p.sigma[0], p.sigma[1], (long long)p.n_samples, p.patchsize,
-                    p.stds_from_mean, p.continuous_estimate);
+                    p.stds_from_mean, p.continuous_estimate, p.circular_mask_diameter);
         std::printf("cpu_blockwise=%d\nprecision=%s\nweight=%s\nseed=%llu\n", p.cpu_blockwise,
                     precision_name(p.precision), p.weight_fn == INLM_WEIGHT_EXP ? "exp" : "pade",
                     (unsigned long long)p.sample_seed);

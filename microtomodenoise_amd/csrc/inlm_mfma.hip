@@ -1240,6 +1240,343 @@ void filter_mfma(MfmaArgs<TS> a)
 }
 
 // ---------------------------------------------------------------------------
+// Block form for patch radius 2 (FP16 and BF16 storage, FP16 D planes).
+//
+// The patch distance of a 4x4 block of outputs in (x, y) at one z is, per
+// patch plane dz, a 2D correlation of that plane of D with the plane's
+// weights.  Every output of a block reads the plane's rows y0-2 .. y0+5 at
+// columns x0-2 .. x0+5 (a disk of radius 2 around a 4x4 square fits in 8
+// columns), so one v_mfma_f32_16x16x32_f16 covers four such 8-column rows
+// (one per lane group) of 16 blocks:
+//   A[m][k]  m = output (bx, by) of a block, k = 8 q + c: the weight of
+//            input column x0-2+c of lane group q's row, (dx, dy, dz) =
+//            (c-2-bx, row-by, dz), zero outside the patch
+//   B[k][n]  the D plane at block n, lane group q's row, column x0-2+c
+//   C[m][n] += A . B
+// Seven MFMAs cover the 33 taps: plane dz=0 rows -2..5 (two), planes
+// dz=+-1 rows -1..2 (one each, equal weights), their rows 3 and 4 together
+// (one), planes dz=+-2 rows 0..3 (one each, equal weights).  The row form
+// (filter_mfma) takes eleven, since its 16-wide rows leave 12 of 32 K slots
+// unused for a 5-tap row and 14 for a 3-tap row; the seven operand reads
+// here replace eleven.
+//
+// A workgroup of 8 wavefronts owns a 16x16x8 output tile and stages D over
+// the tile and its patch halo, 20x20x12, in LDS (double buffered, one
+// barrier per offset).  Wave w covers x quarter w & 3 (4 columns) and z half
+// w >> 2 (4 slices); its 16 blocks are 4 y-blocks of 4 rows by 4 slices.
+// Lane l reads block n = l % 16 and holds, after the MFMA, outputs
+// x0..x0+3 of its row by = l / 16.  LDS rows are 48 bytes apart and slices
+// 992 bytes (62 16-byte units, 2 mod 4): the 16-lane groups in which gfx950
+// serves a ds_read_b128 then hit 64 distinct banks for every read of the
+// plan.
+// ---------------------------------------------------------------------------
+namespace b4 {
+constexpr int TX = 16, TY = 16, TZ = 8;
+constexpr int RX = TX + 4, RY = TY + 4, RZ = TZ + 4;  // staged region
+constexpr int ROW = 24;                               // LDS row pitch, halves
+constexpr int SLICE = 496;                            // LDS slice pitch, halves
+constexpr int NROW = RY * RZ;                         // region rows
+constexpr int N8 = 2 * NROW;                          // 8-column chunks at columns 0 and 8
+constexpr int N4 = NROW;                              // 4-column chunks at column 16
+constexpr int PLANE = RZ * SLICE;
+constexpr int SPARE8 = PLANE;  // staging slots past N8 (N4) write behind the plane
+constexpr int SPARE4 = SPARE8 + 8 * (NTM - N8);
+constexpr int LDS = SPARE4 + 4 * (NTM - N4);  // halves per buffer
+static_assert(N8 <= NTM && N4 <= NTM, "one staging slot of each width per thread");
+static_assert(RX == 20, "region rows are staged as columns 0-7, 8-15 and 16-19");
+constexpr int NOPS = 7, NCLS = 5;
+// plane and row of the K group q of MFMA o; o = 4 holds rows 3, 4 of
+// plane +1 (q < 2) and of plane -1 (q >= 2)
+__host__ __device__ constexpr int op_dz(int o, int q)
+{
+    return o <= 1 ? 0 : o == 2 ? 1 : o == 3 ? -1 : o == 4 ? (q < 2 ? 1 : -1) : o == 5 ? 2 : -2;
+}
+__host__ __device__ constexpr int op_row(int o, int q)
+{
+    return o == 0 ? q - 2 : o == 1 ? q + 2 : o <= 3 ? q - 1 : o == 4 ? (q < 2 ? q + 3 : q + 1) : q;
+}
+// operations with equal weights share an A operand (the weights depend on |dz|)
+constexpr int kCls[NOPS] = {0, 1, 2, 2, 3, 4, 4};
+constexpr int kClsOp[NCLS] = {0, 1, 2, 4, 5};
+// LDS offset of MFMA o relative to a lane's row of K group 0 (uniform except o = 4)
+constexpr int kOff[NOPS] = {-2 * ROW, 2 * ROW, SLICE - ROW, -SLICE - ROW, 0, 2 * SLICE, -2 * SLICE};
+}  // namespace b4
+
+template <class TS>
+__device__ __forceinline__ float b4_weight(const MfmaArgs<TS> &a, int dx, int dy, int dz)
+{
+    constexpr const MRows<2> &rows = MRowsOf<2>::value;
+    const int ady = dy < 0 ? -dy : dy, adz = dz < 0 ? -dz : dz;
+    float w = 0.f;
+#pragma unroll
+    for (int p = 0; p < rows.npat; p++)
+        if (rows.pdy[p] == ady && rows.pdz[p] == adz && dx >= -rows.pj[p] && dx <= rows.pj[p]) w = a.band[p][dx + 2];
+    return w;
+}
+
+template <class TS>
+__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) void filter_mfma_b4(MfmaArgs<TS> a)
+{
+    using ST8 = Stager<8, TS, _Float16>;
+    using ST4 = Stager<4, TS, _Float16>;
+    typedef typename ST8::chunk chunk8;
+    typedef typename ST4::chunk chunk4;
+    __shared__ __attribute__((aligned(16))) _Float16 lds[2][b4::LDS];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wg = mxcd_swizzle(blockIdx.x, gridDim.x);
+    const int bxi = wg % a.ntx;
+    const int rem = wg / a.ntx;
+    const int byi = rem % a.nty;
+    const int bzi = rem / a.nty;
+    const int X0 = bxi * b4::TX, Y0 = byi * b4::TY, Z0 = bzi * b4::TZ;
+    const FilterGeom g = a.g;
+    const long sp = (long)g.nxp * g.nyp;
+
+    for (int i = t; i < (int)(sizeof(lds) / 4); i += NTM) reinterpret_cast<unsigned *>(&lds[0][0])[i] = 0u;
+
+    // A operands: lane l holds row m = l % 16 (output bx = m & 3, by = m >> 2)
+    // of K group q = l / 16
+    const int q = lane >> 4;
+    half8 aop[b4::NCLS];
+    {
+        const int m = lane & 15, mx = m & 3, my = m >> 2;
+#pragma unroll
+        for (int c = 0; c < b4::NCLS; c++) {
+            const int o = b4::kClsOp[c];
+            const int dz = b4::op_dz(o, q), row = b4::op_row(o, q);
+#pragma unroll
+            for (int i = 0; i < 8; i++) aop[c][i] = (_Float16)b4_weight(a, i - 2 - mx, row - my, dz);
+        }
+    }
+
+    // Scaled FP16 planes: D * f^2 clamped to [0, 1] by the squaring
+    // instruction, f = 2^m per tile with f^2 <= wmin / (4 * 3.56648 / |mm|)
+    // and |mm| the smallest over the tile's slices: a clamped tap alone adds
+    // four times the tile's largest cutoff distance, so a patch with a
+    // clamped tap keeps weight exactly zero.
+    float pf;
+    {
+        float smax = 0.f;
+#pragma unroll
+        for (int zz = 0; zz < b4::TZ; zz++) smax = __builtin_fmaxf(smax, a.sigma[Z0 + zz < g.nz ? Z0 + zz : g.nz - 1]);
+        const float x = __builtin_fabsf((-1.f / (smax * smax * a.beta)) * a.mscale) * a.wmin * (1.f / (4.f * 3.56648f));
+        int e = (int)((__builtin_bit_cast(unsigned, x) >> 23) & 0xffu) - 127;
+        e = e < -24 ? -24 : e > 16 ? 16 : e;
+        const int mexp = e >> 1;
+        pf = __builtin_bit_cast(float, (unsigned)(127 + mexp) << 23);
+    }
+
+    // Staging: region row r = rz * 20 + ry (slice Z0-2+rz, row Y0-2+ry) as
+    // 8-column chunks at columns 0 and 8 (thread t: chunk t) and a 4-column
+    // chunk at column 16 (thread t: row t); column c holds x = X0-2+c.
+    const TS *Pt = a.P + ((long)(Z0 - 2 + g.pz) * sp + (long)(Y0 - 2 + g.py) * g.nxp + (X0 - 2 + g.px));
+    ST8 stg8;
+    ST4 stg4;
+    const _Float16 fh = (_Float16)pf;
+    stg8.f = half2v{fh, fh};
+    stg4.f = half2v{fh, fh};
+    int coff8, loff8, coff4, loff4;
+    if (t < b4::N8) {
+        const int row = t >> 1, col = 8 * (t & 1), rz = row / b4::RY, ry = row % b4::RY;
+        coff8 = (int)(rz * sp + (long)ry * g.nxp + col);
+        loff8 = rz * b4::SLICE + ry * b4::ROW + col;
+    } else {  // spare slots re-read chunk 0
+        coff8 = 0;
+        loff8 = b4::SPARE8 + 8 * (t - b4::N8);
+    }
+    if (t < b4::N4) {
+        const int rz = t / b4::RY, ry = t % b4::RY;
+        coff4 = (int)(rz * sp + (long)ry * g.nxp + 16);
+        loff4 = rz * b4::SLICE + ry * b4::ROW + 16;
+    } else {
+        coff4 = 0;
+        loff4 = b4::SPARE4 + 4 * (t - b4::N4);
+    }
+    // this lane's block and outputs
+    const int xq = wave & 3, zh = wave >> 2;
+    const int n = lane & 15, yb = n & 3, zz = n >> 2;
+    const int xb = X0 + 4 * xq;
+    const int y = Y0 + 4 * yb + q;
+    const int z = Z0 + 4 * zh + zz;
+    const TS *Rt = a.R + ((long)(Z0 + g.pz) * sp + (long)(Y0 + g.py) * g.nxp + (X0 + g.px));
+    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(y - Y0) * g.nxp + (xb - X0)) * sizeof(TS));
+    const TS *Rv = reinterpret_cast<const TS *>(reinterpret_cast<const char *>(Rt) + rbyte);
+    const int zc = z < g.nz ? z : g.nz - 1;
+    const float sig = a.sigma[zc];
+    const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
+    // D operand: block n, K group q's row; MFMA o adds kOff[o], o = 4 its own
+    // per-group offset
+    const int bbase = (4 * zh + zz + 2) * b4::SLICE + (4 * yb + 2 + q) * b4::ROW + 4 * xq;
+    const int b4base = bbase + (q < 2 ? b4::SLICE + 3 * b4::ROW : -b4::SLICE + b4::ROW);
+
+    float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
+    ushort2w wmax2[2] = {{0, 0}, {0, 0}};
+    const float dinv = 1.f / (pf * pf);
+    const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f) * dinv);
+    const half2w msu = {msu1, msu1};
+
+    // Buffer resources per volume and tile from the smallest search offset;
+    // an odd (px + lin) reads the copy shifted by one element, one element
+    // earlier, so every load is 4-byte aligned.
+    const TS *Pb = Pt + a.min_lin;
+    const TS *Rb = Rt + a.min_lin;
+    const TS *Pb1 = Pb + (a.shifted - 1);
+    const TS *Rb1 = Rb + (a.shifted - 1);
+    auto records = [](long bytes) __attribute__((always_inline)) {
+        const unsigned long b = (unsigned long)bytes;
+        return (b >> 31) != 0 ? 0x7fffffff : (int)b;
+    };
+    const int nP = records((long)(a.P_end - Pb) * (long)sizeof(TS));
+    const int nR = records((long)(a.R_end - Rb) * (long)sizeof(TS));
+    auto odd_of = [&](int lin) __attribute__((always_inline)) { return ((lin + g.px) & 1) != 0; };
+    auto rsrc_p = [&](int lin) __attribute__((always_inline)) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<TS *>(odd_of(lin) ? Pb1 : Pb), (short)0, nP, kRsrcWord3);
+    };
+    auto rsrc_r = [&](int lin) __attribute__((always_inline)) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<TS *>(odd_of(lin) ? Rb1 : Rb), (short)0, nR, kRsrcWord3);
+    };
+    const int min_lin = a.min_lin;
+    const float one = a.one;
+    asm volatile("" ::"s"(min_lin), "s"(one));
+    auto soff_of = [&](int lin) __attribute__((always_inline)) {
+        return (unsigned)((lin - min_lin) * (int)sizeof(TS));
+    };
+    chunk8 pn8;
+    chunk4 pn4;
+    auto load_chunks = [&](int lin) __attribute__((always_inline)) {
+        const rsrc_t r = rsrc_p(lin);
+        const unsigned so = soff_of(lin);
+        stg8.load(pn8, r, (unsigned)(coff8 * (int)sizeof(TS)), so);
+        stg4.load(pn4, r, (unsigned)(coff4 * (int)sizeof(TS)), so);
+    };
+    // the chunks at the tile's own positions (the P(y) side of every
+    // difference, search offset 0), scaled by -f, in registers
+    chunk8 p0r8;
+    chunk4 p0r4;
+    load_chunks(0);
+    __builtin_memcpy(&p0r8, &pn8, sizeof(p0r8));
+    __builtin_memcpy(&p0r4, &pn4, sizeof(p0r4));
+    stg8.own_scaled(p0r8);
+    stg4.own_scaled(p0r4);
+    auto stage = [&](_Float16 *buf) __attribute__((always_inline)) {
+        stg8.store_scaled(pn8, p0r8, buf + loff8);
+        stg4.store_scaled(pn4, p0r4, buf + loff4);
+    };
+    struct RVals {
+        TS v[4];
+    };
+    auto load_r = [&](RVals &rv, int lin) __attribute__((always_inline)) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc_r(lin), rbyte, soff_of(lin), 0);
+        __builtin_memcpy(rv.v, &v, 8);
+    };
+    auto accumulate = [&](const floatx4 &d, const RVals &rv) __attribute__((always_inline)) {
+        half2w w[2];
+        packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
+                       __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float wi = (float)w[i >> 1][i & 1];
+            fv[i] = __builtin_fmaf(wi, (float)rv.v[i], fv[i]);
+            // times a runtime 1.0, so the sum stays one mixed-precision FMA
+            fw[i] = __builtin_fmaf(wi, one, fw[i]);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) wmax2[j] = __builtin_elementwise_max(wmax2[j], __builtin_bit_cast(ushort2w, w[j]));
+    };
+
+    const int nS = a.n_search;
+    auto lin_at = [&](int k) __attribute__((always_inline)) { return a.search_full[k < nS ? k : nS - 1].lin; };
+
+    // Software pipeline as in filter_mfma: step k runs the MFMA chain of
+    // offset k right after the barrier, with the weights of offset k-1 and
+    // the staging arithmetic of offset k+1 interleaved, writes that plane to
+    // the idle buffer and loads the chunks of offset k+2.
+    __syncthreads();
+    load_chunks(lin_at(0));
+    stage(lds[0]);
+    load_chunks(lin_at(1));
+    __syncthreads();
+
+    floatx4 dA = {0.f, 0.f, 0.f, 0.f};
+    floatx4 dB = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+    RVals rA, rB;
+#pragma unroll
+    for (int i = 0; i < 4; i++) rA.v[i] = rB.v[i] = TS(0.f);
+    constexpr int kBlock = 62;
+    int linv = 0, kb = 0;
+    auto step = [&](int k, const _Float16 *rd, _Float16 *st, floatx4 &dout, RVals &rout, const floatx4 &din,
+                    const RVals &rin) __attribute__((always_inline)) {
+        const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
+        const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
+        load_r(rout, l0);
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        constexpr int NR = b4::NOPS;
+        constexpr int KD = 2;
+        auto b_of = [&](auto oi) __attribute__((always_inline)) {
+            constexpr int o = decltype(oi)::value;
+            if constexpr (o == 4)
+                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + b4base, 8));
+            else
+                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + bbase + b4::kOff[o], 8));
+        };
+        half8 ab[KD];
+        mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = b_of(ri); });
+        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+            constexpr int r = decltype(ri)::value;
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[b4::kCls[r]], ab[r % KD], acc, 0, 0, 0);
+            if constexpr (r + KD < NR) ab[r % KD] = b_of(std::integral_constant<int, r + KD>{});
+        });
+        accumulate(din, rin);
+        stage(st);
+        constexpr int NV = (2 * kWeightValuPacked + kStageValu<TS, _Float16> * 3 + NR - 2) / (NR - 1);
+        __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
+        mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
+            __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
+            if constexpr (decltype(ri)::value + KD < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);
+            if constexpr (decltype(ri)::value >= 1) __builtin_amdgcn_sched_group_barrier(0x0002, NV, 0);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+        load_chunks(l2);
+        __syncthreads();
+        dout = acc;
+    };
+    for (kb = 0; kb < nS; kb += kBlock) {
+        linv = lin_at(kb + lane);
+        const int ke = kb + kBlock < nS ? kb + kBlock : nS;
+        for (int k = kb; k < ke; k += 2) {
+            step(k, lds[0], lds[1], dA, rA, dB, rB);
+            step(k + 1, lds[1], lds[0], dB, rB, dA, rA);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+    if ((nS & 1) == 0) accumulate(dB, rB);
+
+    float wmax[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) wmax[i] = (float)__builtin_bit_cast(half2w, wmax2[i >> 1])[i & 1];
+    if (y < g.ny && z < g.nz) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int x = xb + i;
+            if (x < g.nx) {
+                const long o = ((long)z * g.ny + y) * g.nx + x;
+                float res = a.raw[o];
+                if (wmax[i] > 0.f) {
+                    const double r0 = (double)(float)Rv[i];
+                    const double v = ((double)fv[i] + (double)wmax[i] * r0) / ((double)fw[i] + (double)wmax[i]);
+                    res = (float)(a.shift + v * a.inv_scale);
+                }
+                a.out[o] = res;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Normalising reflect-pad: padded v_s = (v - shift) * scale in the storage
 // type, mirror without repeating the edge (iternlm_prepare.cpp:6-48).
 // ---------------------------------------------------------------------------
@@ -1429,7 +1766,8 @@ template <int R, class TS, class TD, int WFN>
 hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shifted, const float *raw, float *out,
                          const float *sigma, float beta, float mscale, double shift, double inv_scale,
                          const SearchEntry *search, const SearchEntry *search_full, int n_search, int min_lin,
-                         const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
+                         const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, bool row_form,
+                         hipStream_t st)
 {
     if constexpr (sizeof(TS) == 2) {
         // 16-bit loads stay 4-byte aligned only with an even row pitch and
@@ -1487,6 +1825,18 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
         if (iso) fp8_tables_plan<R, true>(codes, a);
         else fp8_tables_plan<R, false>(codes, a);
     }
+    // the block form: full search at patch radius 2, 16-bit storage, FP16
+    // planes, Pade weights
+    if constexpr (R == 2 && sizeof(TS) == 2 && std::is_same<TD, _Float16>::value && WFN == 0) {
+        if (!ol && !row_form) {
+            a.ntx = (g.nx + b4::TX - 1) / b4::TX;
+            a.nty = (g.ny + b4::TY - 1) / b4::TY;
+            a.ntz = (g.nz + b4::TZ - 1) / b4::TZ;
+            const long nb = (long)a.ntx * a.nty * a.ntz;
+            hipLaunchKernelGGL((filter_mfma_b4<TS>), dim3((unsigned)nb), dim3(NTM), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((unsigned)nwg), block(NTM);
     if (ol && iso) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, true>), grid, block, 0, st, a);
     else if (ol) hipLaunchKernelGGL((filter_mfma<R, TS, TD, WFN, true, false>), grid, block, 0, st, a);
@@ -1500,25 +1850,22 @@ hipError_t launch_mfma_form(int form, const void *Rs, const void *Ps, size_t ele
                             float *out,
                             const float *sigma, float beta, float mscale, double shift, double inv_scale,
                             const SearchEntry *search, const SearchEntry *search_full, int n_search, int min_lin,
-                            const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, hipStream_t st)
+                            const TapTable &taps, const FilterGeom &g, const OffsetLists *ol, bool row_form,
+                            hipStream_t st)
 {
     switch (form) {
     case FORM_FP32_FAST:
         return launch_mfma_t<R, float, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                      inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
-                                                      st);
+                                                      inv_scale, search, search_full, n_search, min_lin, taps, g, ol, row_form, st);
     case FORM_FP16:
         return launch_mfma_t<R, _Float16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                         inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
-                                                      st);
+                                                         inv_scale, search, search_full, n_search, min_lin, taps, g, ol, row_form, st);
     case FORM_FP16_FP8:
         return launch_mfma_t<R, _Float16, f8_t, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
-                                                      st);
+                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol, row_form, st);
     case FORM_BF16:
         return launch_mfma_t<R, __bf16, _Float16, WFN>(Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift,
-                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol,
-                                                      st);
+                                                     inv_scale, search, search_full, n_search, min_lin, taps, g, ol, row_form, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1540,14 +1887,14 @@ hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, c
                               long shifted, const float *raw, float *out, const float *sigma, float beta, float mscale,
                               double shift, double inv_scale, const SearchEntry *search,
                               const SearchEntry *search_full, int n_search, int min_lin, const TapTable &taps,
-                              const FilterGeom &g, hipStream_t st, const OffsetLists *ol)
+                              const FilterGeom &g, hipStream_t st, const OffsetLists *ol, bool row_form)
 {
     if (min_lin > 0) return hipErrorInvalidValue;
 #define INLM_MF(RR)                                                                                               \
     return weight_fn == 0 ? launch_mfma_form<RR, 0>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, search_full, n_search, min_lin, taps, g, ol, st)                                 \
+                                                    search, search_full, n_search, min_lin, taps, g, ol, row_form, st)                       \
                           : launch_mfma_form<RR, 1>(form, Rs, Ps, elems, shifted, raw, out, sigma, beta, mscale, shift, inv_scale, \
-                                                    search, search_full, n_search, min_lin, taps, g, ol, st)
+                                                    search, search_full, n_search, min_lin, taps, g, ol, row_form, st)
     switch (rp) {
     case 1: INLM_MF(1);
     case 2: INLM_MF(2);

@@ -129,32 +129,37 @@ inlm_status denoise_rgb(const inlm_params *pin, const inlm_rgb_options *c, const
     RGB_HIP(hipMemcpyAsync(Ud, host.data(), 3 * npix * sizeof(float), hipMemcpyHostToDevice, st));
     for (int ch = 0; ch < 3; ch++) RGB_HIP(launch_pad_reflect(Ud + ch * npix, Rd + ch * padn, g, st));
 
-    // noise level per channel (noiselevel.cpp:96-124): one sample list for
+    // noise level per channel (noiselevel.cpp:96-184): one sample list for
     // every estimate, two clusters over the patch deviations of each channel
+    // (the z-adaptive mode is global for a 2D image), or the semimanual mode
+    // with a reference sample per channel
     std::vector<int64_t> samples;
     const int64_t ns = p.noise_mode == INLM_NOISE_MANUAL ? 0 : p.n_samples;
     if (ns > 0) {
         samples.resize((size_t)ns);
-        if (!draw_samples(sh, ns, p.patchsize, p.sample_seed, samples.data()))
-            return rgb_fail(INLM_BAD_ARG, "image smaller than the noise patch");
+        if (!draw_samples(sh, ns, p.patchsize, p.sample_seed, samples.data(), p.circular_mask_diameter))
+            return rgb_fail(INLM_BAD_ARG, "no noise sample fits the image (or the circular mask)");
         RGB_HIP(hipMalloc(&bSm.p, (size_t)ns * sizeof(int64_t)));
         RGB_HIP(hipMalloc(&bSd.p, (size_t)ns * sizeof(double)));
         RGB_HIP(hipMemcpyAsync(bSm.p, samples.data(), (size_t)ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
     }
     std::vector<double> sd((size_t)std::max<int64_t>(ns, 1));
+    inlm_params pc = p;
+    if (pc.noise_mode == INLM_NOISE_ZADAPTIVE) pc.noise_mode = INLM_NOISE_GLOBAL;
+    int64_t semi_ref[3] = {-1, -1, -1};
     auto estimate = [&](const float *chans_dev, int which, float sig[3]) -> inlm_status {
-        if (ns == 0) {
-            for (int ch = 0; ch < 3; ch++) sig[ch] = p.sigma[which == 0 ? 0 : 1];
-        } else {
-            for (int ch = 0; ch < 3; ch++) {
+        const int one_slice[3] = {nx, ny, 1};
+        for (int ch = 0; ch < 3; ch++) {
+            if (ns > 0) {
                 RGB_HIP(launch_patchstd(chans_dev + ch * npix, nx, (int64_t)npix, static_cast<int64_t *>(bSm.p), ns,
                                         nx, (int64_t)npix, p.patchsize, static_cast<double *>(bSd.p), st));
                 RGB_HIP(hipMemcpyAsync(sd.data(), bSd.p, (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, st));
                 RGB_HIP(hipStreamSynchronize(st));
-                double mean = 0, stdev = 0;
-                create2clusters(sd.data(), (int)ns, 0, (int)ns - 1, mean, stdev);
-                sig[ch] = (float)(mean + p.stds_from_mean * stdev);
             }
+            std::string err;
+            const inlm_status r = sigma_from_std(pc, one_slice, samples.data(), sd.data(), ns, which, semi_ref[ch],
+                                                 &sig[ch], err);
+            if (r != INLM_OK) return rgb_fail(r, err);
         }
         if (c->average_channelsigma) sig[0] = sig[1] = sig[2] = (sig[0] + sig[1] + sig[2]) / 3.f;
         return INLM_OK;

@@ -123,15 +123,35 @@ static uint64_t splitmix64(uint64_t &s)
     return z ^ (z >> 31);
 }
 
-bool draw_samples(const int shape[3], int64_t n, int patchsize, uint64_t seed, int64_t *out)
+bool draw_samples(const int shape[3], int64_t n, int patchsize, uint64_t seed, int64_t *out, int mask_diameter)
 {
     const int nx = shape[0], ny = shape[1], nz = shape[2];
     if (nx <= patchsize || ny <= patchsize || nz < 1 || n < 0) return false;
+    // Circular field of view (noiselevel.cpp:204-224): the corner (x, y) of
+    // a sample is redrawn while (x - nx/2)^2 + (y - ny/2)^2 exceeds
+    // (diameter/2 - sqrt(2 ps^2))^2, the square taken in float as the
+    // reference does (a negative radius therefore squares to a positive one).
+    float maxr2 = 0.f;
+    const int cx = nx / 2, cy = ny / 2;
+    auto r2 = [&](int x, int y) { return (float)((x - cx) * (x - cx) + (y - cy) * (y - cy)); };
+    if (mask_diameter > 0) {
+        maxr2 = (float)((double)(mask_diameter / 2.f) - std::sqrt((double)(2 * patchsize * patchsize)));
+        maxr2 *= maxr2;
+        // the reference loops for ever when no corner of the sampling range
+        // lies inside the circle; the corner closest to the centre decides
+        const int bx = std::min(cx, nx - patchsize - 1), by = std::min(cy, ny - patchsize - 1);
+        if (r2(bx, by) > maxr2) return false;
+    }
     uint64_t st = seed;
     for (int64_t i = 0; i < n; i++) {
         int x = (int)(splitmix64(st) % (uint64_t)(nx - patchsize));
         int y = (int)(splitmix64(st) % (uint64_t)(ny - patchsize));
-        int z = (int)(splitmix64(st) % (uint64_t)nz);
+        const int z = (int)(splitmix64(st) % (uint64_t)nz);
+        if (mask_diameter > 0)
+            while (r2(x, y) > maxr2) {
+                x = (int)(splitmix64(st) % (uint64_t)(nx - patchsize));
+                y = (int)(splitmix64(st) % (uint64_t)(ny - patchsize));
+            }
         out[i] = (int64_t)z * nx * ny + (int64_t)y * nx + x;
     }
     std::sort(out, out + n);

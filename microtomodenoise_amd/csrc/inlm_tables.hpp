@@ -37,8 +37,11 @@ bool cpu_weight_table(const int rp[3], int nslices, float z_aniso, int nz_padded
                       std::vector<float> &out, std::string &err);
 
 // SplitMix64 sample positions with the distribution of
-// NoiseLevel::drawsamples (noiselevel.cpp:192-234), sorted.
-bool draw_samples(const int shape[3], int64_t n, int patchsize, uint64_t seed, int64_t *out);
+// NoiseLevel::drawsamples (noiselevel.cpp:192-234), sorted; mask_diameter > 0
+// restricts the sample corners to the circular field of view of
+// noiselevel.cpp:204-224.  False when the volume is smaller than the patch or
+// no corner of the sampling range lies inside the circle.
+bool draw_samples(const int shape[3], int64_t n, int patchsize, uint64_t seed, int64_t *out, int mask_diameter = 0);
 
 // Two-cluster hard EM (noiselevel.cpp:366-528) in FP64 over values[fs..ls].
 void create2clusters(const double *values, int n, int fs, int ls, double &mean, double &std);

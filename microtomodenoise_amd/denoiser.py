@@ -4,10 +4,12 @@
 (iterNLM_v0.3/denoise_parameters.h:9-74) with the same field names and
 defaults.  ``IterativeNLM_GPU`` keeps the method names and argument meaning of
 ``denoise::IterativeNLM_GPU`` (Denoiser/iternlm_gpu.h:20-56):
-``configure_device`` returns the number of slices it can hold (the whole
-volume: 288 GB of HBM hold every supported shape), -1 when no GPU is
-visible and -2 on a HIP error, like the reference (iternlm_gpu.cu:1443-1444,
-:1608-1618).  Unlike the reference, failures of the other methods raise
+``configure_device`` returns the number of slices one slab may hold
+(shape[2] when the whole volume fits the free device memory, or
+``gpu.memory_limit`` bytes), -1 when no GPU is visible and -2 on a HIP
+error, like the reference (iternlm_gpu.cu:1443-1444, :1528-1566,
+:1608-1618); below shape[2] the caller takes
+``Run_GaussianNoise_GPUBlocks``.  Unlike the reference, failures of the other methods raise
 ``InlmError`` instead of printing and returning.
 
 Volumes are numpy float32 arrays of shape ``(nz, ny, nx)`` (x fastest); the
@@ -26,7 +28,7 @@ from . import _capi as capi
 
 @dataclass
 class NoiseLevel:
-    mode: str = "z-adaptive"  # "z-adaptive", "global" ("auto"), "manual"
+    mode: str = "z-adaptive"  # "z-adaptive", "global" ("auto"), "manual", "semimanual"
     zwindow: int = 100
     sigma: List[float] = field(default_factory=lambda: [0.0, 0.0])
     n_samples: int = 100000
@@ -50,6 +52,8 @@ class GPU:
     threadsPerBlock: int = 128
     memory_buffer: int = 0
     blockwise_host: bool = False
+    # device bytes configure_device may plan for (0: the free device memory)
+    memory_limit: int = 0
 
 
 @dataclass
@@ -87,6 +91,8 @@ class DenoiseParameters:
             p.noise_mode = capi.NOISE_ZADAPTIVE
         elif mode == "manual":
             p.noise_mode = capi.NOISE_MANUAL
+        elif mode == "semimanual":
+            p.noise_mode = capi.NOISE_SEMIMANUAL
         else:
             raise ValueError(f"unsupported noise mode {mode!r}")
         p.zwindow = self.noiselevel.zwindow
@@ -95,6 +101,7 @@ class DenoiseParameters:
         p.patchsize = self.noiselevel.patchsize
         p.stds_from_mean = self.noiselevel.stds_from_mean
         p.continuous_estimate = int(self.noiselevel.continuous_estimate)
+        p.circular_mask_diameter = int(self.noiselevel.circular_mask_diameter)
         p.sample_seed = int(self.sample_seed)
         p.cpu_blockwise = int(self.cpu.blockwise)
         p.precision = {"fp32": capi.PREC_FP32, "fp32_fast": capi.PREC_FP32_FAST, "bf16": capi.PREC_BF16,
@@ -116,6 +123,7 @@ class IterativeNLM_GPU:
         self.resumed = False
         self._h = C.c_void_p()
         self._shape = None
+        self._sigma = None
 
     # -- iternlm_gpu.cu:1440-1623 -------------------------------------------
     def configure_device(self, shape, params: DenoiseParameters) -> int:
@@ -124,7 +132,12 @@ class IterativeNLM_GPU:
             params.gpu.n_gpus = 0
             return -1
         sh = np.asarray(shape, np.int32)
-        st = L.inlm_create(C.byref(params.to_c()), capi.ptr(sh, C.c_int32), params.gpu.deviceID, C.byref(self._h))
+        cp = params.to_c()
+        n = C.c_int32()
+        st = L.inlm_capacity_slices(C.byref(cp), capi.ptr(sh, C.c_int32), params.gpu.deviceID,
+                                    int(params.gpu.memory_limit), C.byref(n))
+        if st == capi.INLM_OK and n.value >= int(sh[2]):
+            st = L.inlm_create(C.byref(cp), capi.ptr(sh, C.c_int32), params.gpu.deviceID, C.byref(self._h))
         if st == capi.INLM_NO_DEVICE:
             params.gpu.n_gpus = 0
             return -1
@@ -136,7 +149,9 @@ class IterativeNLM_GPU:
         params.gpu.n_gpus = 1
         self._maxit = int(params.maxiterations)
         self._shape = tuple(int(v) for v in shape)
-        self.n_parallelslices = self._shape[2]
+        self._params = params
+        self._device = params.gpu.deviceID
+        self.n_parallelslices = n.value
         return self.n_parallelslices
 
     def free_device(self):
@@ -156,7 +171,22 @@ class IterativeNLM_GPU:
     # -- iternlm_gpu.cu:2341-2355 -------------------------------------------
     def set_sigma(self, sigmalist, shape=None):
         s = _f32(sigmalist)
-        self._check(capi.lib().inlm_set_sigma(self._h, capi.ptr(s, C.c_float), len(s)))
+        self._sigma = s.copy()
+        if self._h:
+            self._check(capi.lib().inlm_set_sigma(self._h, capi.ptr(s, C.c_float), len(s)))
+
+    # -- iternlm_gpu.cu:1729-1882 -------------------------------------------
+    def Run_GaussianNoise_GPUBlocks(self, iter: int, instack, previous, shape=None, params=None) -> np.ndarray:
+        """One iteration over slabs of n_parallelslices slices on host
+        arrays; returns the new iterate (the reference writes it into
+        `previous`, which this mirror also does when it is a float32 array)."""
+        raw = _f32(instack)
+        prev = _f32(previous) if previous is not None else raw
+        out = iterate_slabs(raw, prev, self._sigma, params or self._params, iter, max(1, self.n_parallelslices),
+                            self._device)
+        if isinstance(previous, np.ndarray) and previous.dtype == np.float32 and previous.shape == out.shape:
+            previous[...] = out
+        return out
 
     # -- iternlm_gpu.cu:1651-1728 -------------------------------------------
     def Run_GaussianNoise(self, iter: int, instack=None, shape=None, params=None):
@@ -291,6 +321,22 @@ def denoise_slabs(instack: np.ndarray, params: DenoiseParameters, slab_slices: i
                               capi.ptr(out, C.c_float), int(slab_slices), capi.ptr(log, C.c_float))
     capi.check(st)
     return out, log[: params.maxiterations]
+
+
+def iterate_slabs(raw: np.ndarray, prev: np.ndarray, sigma, params: DenoiseParameters, iter: int,
+                  slab_slices: int, device: int = 0) -> np.ndarray:
+    """One filter iteration over z-slabs of at most `slab_slices` output
+    slices (inlm_iterate_slabs); prev is ignored at iteration 1."""
+    raw = _f32(raw)
+    prev = _f32(prev)
+    sig = _f32(sigma)
+    nz, ny, nx = raw.shape
+    out = np.empty_like(raw)
+    sh = np.array([nx, ny, nz], np.int32)
+    capi.check(capi.lib().inlm_iterate_slabs(C.byref(params.to_c()), capi.ptr(sh, C.c_int32), device, int(iter),
+                                             capi.ptr(raw, C.c_float), capi.ptr(prev, C.c_float),
+                                             capi.ptr(sig, C.c_float), capi.ptr(out, C.c_float), int(slab_slices)))
+    return out
 
 
 def rgb_options(independent_channels=False, stabilize_variance=True, average_channelsigma=True):

@@ -526,6 +526,45 @@ ORC_API int orc_draw_samples(const int shape[3], long long n, int patchsize, uin
     return 0;
 }
 
+/* Circular field of view (noiselevel.cpp:204-224): a sample corner (x, y)
+ * is redrawn while (x - nx/2)^2 + (y - ny/2)^2 > r^2, r = diameter/2 -
+ * sqrt(2 ps^2), the square taken in float.  The reference would loop for
+ * ever when no corner of the sampling range is inside; here a brute-force
+ * scan of the range returns -1 for that case. */
+ORC_API int orc_draw_samples_mask(const int shape[3], long long n, int patchsize, int mask_diameter, uint64_t seed,
+                                  long long *out)
+{
+    if (mask_diameter <= 0) return orc_draw_samples(shape, n, patchsize, seed, out);
+    const int nx = shape[0], ny = shape[1], nz = shape[2];
+    if (nx <= patchsize || ny <= patchsize || nz < 1) return -1;
+    float maxr = (float)((double)((float)mask_diameter / 2.f) - sqrt((double)(2 * patchsize * patchsize)));
+    maxr = maxr * maxr;
+    const int cx = nx / 2, cy = ny / 2;
+    int feasible = 0;
+    for (int y = 0; y < ny - patchsize && !feasible; y++)
+        for (int x = 0; x < nx - patchsize; x++)
+            if ((float)((x - cx) * (x - cx) + (y - cy) * (y - cy)) <= maxr) {
+                feasible = 1;
+                break;
+            }
+    if (!feasible) return -1;
+    uint64_t st = seed;
+    for (long long i = 0; i < n; i++) {
+        int x = (int)(splitmix64(&st) % (uint64_t)(nx - patchsize));
+        int y = (int)(splitmix64(&st) % (uint64_t)(ny - patchsize));
+        int z = (int)(splitmix64(&st) % (uint64_t)nz);
+        float r = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
+        while (r > maxr) {
+            x = (int)(splitmix64(&st) % (uint64_t)(nx - patchsize));
+            y = (int)(splitmix64(&st) % (uint64_t)(ny - patchsize));
+            r = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
+        }
+        out[i] = (long long)z * nx * ny + (long long)y * nx + x;
+    }
+    qsort(out, (size_t)n, sizeof(long long), cmp_ll);
+    return 0;
+}
+
 /* Per-sample 2D patch standard deviation: float mean, then float population
  * variance, then sqrtf (noiselevel.cpp:235-283). */
 ORC_API void orc_patchstd_f32(const float *vol, const int shape[3], const long long *samples, long long n,

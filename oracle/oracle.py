@@ -59,6 +59,7 @@ def orc():
                                           _f32p, C.c_int, _i32p, _f64p]
         L.orc_requant_u16.argtypes = [_f32p, C.c_longlong, _u16p]
         L.orc_draw_samples.argtypes = [_i32p, C.c_longlong, C.c_int, C.c_uint64, _i64p]
+        L.orc_draw_samples_mask.argtypes = [_i32p, C.c_longlong, C.c_int, C.c_int, C.c_uint64, _i64p]
         L.orc_patchstd_f32.argtypes = [_f32p, _i32p, _i64p, C.c_longlong, C.c_int, _f32p]
         L.orc_patchstd_f64.argtypes = [_f32p, _i32p, _i64p, C.c_longlong, C.c_int, _f64p]
         L.orc_create2clusters_f32.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
@@ -117,6 +118,7 @@ class Params:
     n_samples: int = 100000
     patchsize: int = 15
     stds_from_mean: float = 1.0
+    sigma: tuple = (0.0, 0.0)  # manual and semimanual modes
     extra: dict = field(default_factory=dict)
 
     def padding(self):
@@ -321,11 +323,13 @@ def requant_u16(x: np.ndarray) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # Noise estimator
 # ---------------------------------------------------------------------------
-def draw_samples(shape_xyz, n, patchsize=15, seed=0) -> np.ndarray:
+def draw_samples(shape_xyz, n, patchsize=15, seed=0, mask_diameter=0) -> np.ndarray:
+    """drawsamples (noiselevel.cpp:192-234) on SplitMix64(seed); with
+    mask_diameter > 0 the corners stay inside the circular field of view."""
     out = np.zeros(n, np.int64)
-    rc = orc().orc_draw_samples(_i32(shape_xyz), n, patchsize, seed, out)
+    rc = orc().orc_draw_samples_mask(_i32(shape_xyz), n, patchsize, int(mask_diameter), seed, out)
     if rc != 0:
-        raise ValueError("volume too small for the noise patch")
+        raise ValueError("volume too small for the noise patch, or no corner inside the mask")
     return out
 
 
@@ -364,12 +368,28 @@ def zwindow(samples, values, shape_xyz, windowsize=100, precision="f32"):
     return out.reshape(nz, 3)
 
 
-def noiselevel(vol, samples, p: Params, precision="f32") -> np.ndarray:
-    """get_noiselevel (Noise/noiselevel.cpp:9-95) for the global and
-    z-adaptive modes on a sample list: sigma = mean + stds_from_mean * std of
-    the lower cluster."""
+def noiselevel(vol, samples, p: Params, precision="f32", which=0, state=None) -> np.ndarray:
+    """get_noiselevel (Noise/noiselevel.cpp:9-95) on a sample list: global
+    and z-adaptive give sigma = mean + stds_from_mean * std of the lower
+    cluster; manual gives sigma[0] on the first estimate (which=0) and
+    sigma[1] later; semimanual (:56-79) gives sigma[0] first and records in
+    state["ref"] the first sample whose patch std is closest to it, and
+    later that sample's patch std."""
     nz, ny, nx = vol.shape
+    dt = np.float32 if precision == "f32" else np.float64
+    if p.noise_mode == "manual":
+        return np.full(nz, p.sigma[0 if which == 0 else 1], dt)
     ps = patchstd(vol, samples, p.patchsize, precision)
+    if p.noise_mode == "semimanual":
+        if which == 0:
+            best, ref = 1e9, 0
+            for i, v in enumerate(ps):
+                d = abs(float(v) - float(np.float32(p.sigma[0])))
+                if d < best:
+                    best, ref = d, i
+            state["ref"] = ref
+            return np.full(nz, p.sigma[0], dt)
+        return np.full(nz, ps[state["ref"]], dt)
     if p.noise_mode == "z-adaptive":
         zw = zwindow(samples, ps, (nx, ny, nz), p.zwindow, precision)
         if precision == "f32":
@@ -394,13 +414,14 @@ def iternlm(raw: np.ndarray, p: Params, *, sigma0=None, sigma1=None, samples=Non
     with alpha for iterations >= 2.  Fixed sigma lists may be injected."""
     raw = np.ascontiguousarray(raw, np.float32)
     nz, ny, nx = raw.shape
+    state = {}
     if sigma0 is None:
-        sigma0 = noiselevel(raw, samples, p, precision)
+        sigma0 = noiselevel(raw, samples, p, precision, which=0, state=state)
     prev = raw
     out = filter_iteration(raw, prev, sigma0, p, precision="f32", nthreads=nthreads)
     if p.maxiterations > 1:
         if sigma1 is None:
-            s1 = noiselevel(out, samples, p, precision)
+            s1 = noiselevel(out, samples, p, precision, which=1, state=state)
             sigma1 = blend_sigma(sigma0, s1, p.alpha)
         for _ in range(2, p.maxiterations + 1):
             out = filter_iteration(raw, out, sigma1, p, precision="f32", nthreads=nthreads)

@@ -62,6 +62,10 @@ def test_defaults_are_the_reference_defaults():
     (("x", "-nsamples", "5000", "-noisepatch", "9", "-noiseshift", "2"),
      {"n_samples": "5000", "patchsize": "9", "stds_from_mean": "2"}),
     (("x", "--continuous"), {"continuous": "1"}),
+    (("x", "-noise", "semimanual", "-s", "300"), {"noise": "semimanual", "sigma": "300,300"}),
+    (("x", "-noise", "auto"), {"noise": "global"}),
+    (("x", "-mask", "900"), {"mask_diameter": "900"}),
+    (("x", "-mask_diameter", "0"), {"mask_diameter": "0"}),
     (("x", "--noblocks"), {"cpu_blockwise": "0"}),
     (("x", "--noblocks", "--blocks"), {"cpu_blockwise": "1"}),
     (("x", "-gpu0", "2"), {"device": "2"}),
@@ -96,8 +100,7 @@ def test_first_positional_argument_is_the_input():
 
 @pytest.mark.parametrize("args,msg", [
     (("x", "-search"), "missing value"),
-    (("x", "-noise", "semimanual"), "noise mode"),
-    (("x", "-mask", "900"), "mask"),
+    (("x", "-noise", "median"), "noise estimation mode"),
     (("x", "-precision", "fp8"), "precision"),
     ((), "no input"),
 ])

@@ -246,3 +246,33 @@ def test_fast_forms_anisotropic_patch(gpu_available, form):
     print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
     bmax, brms = FAST_BUDGET[form]
     assert mx <= bmax and rms <= brms, (mx, rms)
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("shape,rs", [((40, 30, 20), (3, 3, 3)), ((41, 35, 13), (4, 4, 2)), ((16, 16, 8), (2, 2, 2)),
+                                      ((67, 18, 9), (7, 7, 5))])
+def test_block_kernel_equals_row_kernel(gpu_available, form, shape, rs):
+    """At patch radius 2 the 16-bit forms run the 4x4-block kernel (seven
+    MFMAs per offset over 16x16x8 tiles); the row kernel (kernel_select 2,
+    eleven MFMAs over 32x8x8 tiles) stages the same FP16 planes with the same
+    per-tile scale, so the two differ only in the order of FP32 sums: ragged
+    shapes, both storage types."""
+    spec = ph.PhantomSpec(shape, seed=53, radii=(2, 6), noise_sigma=1500.0, blur_sigma=0.8, streaks=2,
+                          streak_len=(5, 12))
+    raw = ph.make(spec)
+    nz, ny, nx = raw.shape
+    sig = np.linspace(600.0, 900.0, nz).astype(np.float32)
+    outs = {}
+    for ks in (0, 2):
+        print(f"[gpu] {form} shape {shape} rs {rs} kernel_select {ks}", flush=True)
+        g, _ = make_gpu((nx, ny, nz), rs, (2, 2, 2), precision=form, kernel=ks)
+        g.set_sigma(sig)
+        g.Run_GaussianNoise(1, raw)
+        g.set_sigma(sig * np.float32(0.9))
+        g.Run_GaussianNoise(2)
+        outs[ks] = g.get_result().astype(np.float64)
+        g.free_device()
+    err = np.abs(outs[0] - outs[2]) / spec.noise_sigma
+    mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
+    print(f"[gpu]   block vs row: max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
+    assert mx < 0.02 and rms < 2e-4, (mx, rms)

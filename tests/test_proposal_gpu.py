@@ -232,7 +232,7 @@ def test_learned_lists_capture_more_mass_than_nearest(gpu_available):
     offsets nearest the centre, and the K offsets of highest mean training
     mass), on a held-out phantom: captured true weight mass at equal K
     (strictly more), and PSNR of the pruned FP16 filter against the clean
-    phantom (no worse than the nearest lists, within 0.05 dB)."""
+    phantom (strictly higher than with either list that needs no network)."""
     rs, rp = (7, 7, 7), (2, 2, 2)
     K = 128
     shape = (64, 64, 32)
@@ -287,7 +287,6 @@ def test_learned_lists_capture_more_mass_than_nearest(gpu_available):
     print("[gpu]   PSNR " + ", ".join(f"{k} {v:.3f}" for k, v in ps.items()), flush=True)
     net.close()
     assert cap["learned"] > cap["nearest"] and cap["learned"] > cap["popular"]
-    # at K = 128 of 1326 the pruned PSNRs lie within a few hundredths of a
-    # dB of one another; training sums weight gradients with float atomics,
-    # so the learned lists, and their PSNR, vary slightly from run to run
-    assert ps["learned"] > ps["nearest"] - 0.05 and ps["learned"] < ps["full"]
+    # what the network gains over the lists that need none: more of the true
+    # weight mass kept, and a pruned filter closer to the clean phantom
+    assert ps["learned"] > ps["nearest"] and ps["learned"] > ps["popular"] and ps["learned"] < ps["full"]
