@@ -592,15 +592,19 @@ struct Stager<W, _Float16, _Float16> {
 #pragma unroll
         for (int h = 0; h < W / 2; h++) b[h] = -(b[h] * f);
     }
-    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    __device__ __forceinline__ void scaled(const chunk &pn, const chunk &b, half2v (&out)[W / 2]) const
     {
         const half2v zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
-        half2v out[W / 2];
 #pragma unroll
         for (int h = 0; h < W / 2; h++) {
             const half2v d = __builtin_elementwise_fma(pn[h], f, b[h]);
             out[h] = __builtin_elementwise_min(__builtin_elementwise_max(d * d, zero), one);
         }
+    }
+    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    {
+        half2v out[W / 2];
+        scaled(pn, b, out);
         store_dplane<W>(dst, out);
     }
 };
@@ -644,15 +648,19 @@ struct Stager<W, __bf16, _Float16> {
 #pragma unroll
         for (int h = 0; h < W / 2; h++) b[h] = __builtin_bit_cast(unsigned, -(widen(b[h]) * f));
     }
-    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    __device__ __forceinline__ void scaled(const chunk &pn, const chunk &b, half2v (&out)[W / 2]) const
     {
         const half2v zero = {(_Float16)0.f, (_Float16)0.f}, one = {(_Float16)1.f, (_Float16)1.f};
-        half2v out[W / 2];
 #pragma unroll
         for (int h = 0; h < W / 2; h++) {
             const half2v d = __builtin_elementwise_fma(widen(pn[h]), f, __builtin_bit_cast(half2v, b[h]));
             out[h] = __builtin_elementwise_min(__builtin_elementwise_max(d * d, zero), one);
         }
+    }
+    __device__ __forceinline__ void store_scaled(const chunk &pn, const chunk &b, _Float16 *dst) const
+    {
+        half2v out[W / 2];
+        scaled(pn, b, out);
         store_dplane<W>(dst, out);
     }
 };
@@ -1260,30 +1268,45 @@ void filter_mfma(MfmaArgs<TS> a)
 // unused for a 5-tap row and 14 for a 3-tap row; the seven operand reads
 // here replace eleven.
 //
-// A workgroup of 8 wavefronts owns a 16x16x8 output tile and stages D over
-// the tile and its patch halo, 20x20x12, in LDS (double buffered, one
-// barrier per offset).  Wave w covers x quarter w & 3 (4 columns) and z half
-// w >> 2 (4 slices); its 16 blocks are 4 y-blocks of 4 rows by 4 slices.
-// Lane l reads block n = l % 16 and holds, after the MFMA, outputs
-// x0..x0+3 of its row by = l / 16.  LDS rows are 48 bytes apart and slices
-// 992 bytes (62 16-byte units, 2 mod 4): the 16-lane groups in which gfx950
-// serves a ds_read_b128 then hit 64 distinct banks for every read of the
-// plan.
+// A workgroup of 8 wavefronts owns a 32x8x8 output tile (the row form's
+// tile) and stages D over the tile and its patch halo, 36x12x12, in LDS
+// (double buffered, one barrier per offset).  Wave w computes slice Z0 + w:
+// its 16 blocks are 8 x-blocks by 2 y-blocks, lane l reads block n = l % 16
+// (x-block n % 8, y-block n / 8) and holds, after the MFMA, outputs
+// x0..x0+3 of row by = l / 16.  The lanes of one row then cover 32
+// consecutive x, so a wave's R load touches 8 rows: a layout whose wave
+// spans 4 columns and 64 rows took 2.3 times as long, bound by the cache
+// lines each load instruction touches.
+//
+// LDS layout: a region row is 8 windows of 8 columns, window k holding
+// region columns 4k .. 4k+7, so every block's window (region column 4 xb)
+// is window xb, 16-byte aligned.  Rows are 128 bytes apart and slices 1552
+// bytes (97 16-byte units, 1 mod 16).  Under gfx950's service groups for
+// 16-byte accesses (16 lanes per cycle: {0-3,12-15,20-27}, {4-11,16-19,
+// 28-31} and the same + 32) every read of the plan and every staging write
+// hits 16 distinct 16-byte bank groups: a layout that wrote the shifted
+// half-windows with 8-byte stores spent half its LDS cycles in bank
+// conflicts, 7% slower than the row form.
+//
+// Staging: 9 slots of 64 lanes each cover a 4x4 block of region rows
+// (ry in 4 (s % 3) .. +3, rz in 4 (s / 3) .. +3); lane 4 j + c computes the D
+// values of columns 8c .. 8c+7 of row (ry0 + (j>>1 & 1) + 2 (j>>3),
+// rz0 + (j>>2 & 1) + 2 parity(j & 7)), and lanes with c = 3 also of columns
+// 32-35.  It writes window 2c (its own 8 columns) and window 2c+1 (its upper
+// four and the next lane's lower four, moved by DPP; columns 32-35 for
+// c = 3).  Wave w runs slot w, and wave (workgroup index mod 8) slot 8.
 // ---------------------------------------------------------------------------
+// operand reads in flight ahead of the MFMA that consumes them
+#ifndef INLM_B4_KD
+#define INLM_B4_KD 2
+#endif
 namespace b4 {
-constexpr int TX = 16, TY = 16, TZ = 8;
+constexpr int TX = 32, TY = 8, TZ = 8;
 constexpr int RX = TX + 4, RY = TY + 4, RZ = TZ + 4;  // staged region
-constexpr int ROW = 24;                               // LDS row pitch, halves
-constexpr int SLICE = 496;                            // LDS slice pitch, halves
-constexpr int NROW = RY * RZ;                         // region rows
-constexpr int N8 = 2 * NROW;                          // 8-column chunks at columns 0 and 8
-constexpr int N4 = NROW;                              // 4-column chunks at column 16
-constexpr int PLANE = RZ * SLICE;
-constexpr int SPARE8 = PLANE;  // staging slots past N8 (N4) write behind the plane
-constexpr int SPARE4 = SPARE8 + 8 * (NTM - N8);
-constexpr int LDS = SPARE4 + 4 * (NTM - N4);  // halves per buffer
-static_assert(N8 <= NTM && N4 <= NTM, "one staging slot of each width per thread");
-static_assert(RX == 20, "region rows are staged as columns 0-7, 8-15 and 16-19");
+constexpr int ROW = 64;                               // LDS row pitch, halves (8 windows)
+constexpr int SLICE = RY * ROW + 8;                   // LDS slice pitch, halves (97 16-byte units)
+constexpr int LDS = RZ * SLICE;                       // halves per buffer
+static_assert(RX == 36 && RY % 4 == 0 && RZ % 4 == 0 && (RY / 4) * (RZ / 4) == 9, "staging slots");
 constexpr int NOPS = 7, NCLS = 5;
 // plane and row of the K group q of MFMA o; o = 4 holds rows 3, 4 of
 // plane +1 (q < 2) and of plane -1 (q >= 2)
@@ -1314,8 +1337,9 @@ __device__ __forceinline__ float b4_weight(const MfmaArgs<TS> &a, int dx, int dy
     return w;
 }
 
+// Its loop holds about 90 registers: two workgroups per CU (4 waves per SIMD)
 template <class TS>
-__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) void filter_mfma_b4(MfmaArgs<TS> a)
+__global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) void filter_mfma_b4(MfmaArgs<TS> a)
 {
     using ST8 = Stager<8, TS, _Float16>;
     using ST4 = Stager<4, TS, _Float16>;
@@ -1324,7 +1348,8 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     __shared__ __attribute__((aligned(16))) _Float16 lds[2][b4::LDS];
 
     const int t = threadIdx.x;
-    const int lane = t & 63, wave = t >> 6;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wg = mxcd_swizzle(blockIdx.x, gridDim.x);
     const int bxi = wg % a.ntx;
     const int rem = wg / a.ntx;
@@ -1368,47 +1393,43 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
         pf = __builtin_bit_cast(float, (unsigned)(127 + mexp) << 23);
     }
 
-    // Staging: region row r = rz * 20 + ry (slice Z0-2+rz, row Y0-2+ry) as
-    // 8-column chunks at columns 0 and 8 (thread t: chunk t) and a 4-column
-    // chunk at column 16 (thread t: row t); column c holds x = X0-2+c.
+    // Staging (region row (ry, rz) = slice Z0-2+rz, row Y0-2+ry; region
+    // column c = x X0-2+c): this lane's rows in its slot(s) and column group
     const TS *Pt = a.P + ((long)(Z0 - 2 + g.pz) * sp + (long)(Y0 - 2 + g.py) * g.nxp + (X0 - 2 + g.px));
     ST8 stg8;
     ST4 stg4;
     const _Float16 fh = (_Float16)pf;
     stg8.f = half2v{fh, fh};
     stg4.f = half2v{fh, fh};
-    int coff8, loff8, coff4, loff4;
-    if (t < b4::N8) {
-        const int row = t >> 1, col = 8 * (t & 1), rz = row / b4::RY, ry = row % b4::RY;
-        coff8 = (int)(rz * sp + (long)ry * g.nxp + col);
-        loff8 = rz * b4::SLICE + ry * b4::ROW + col;
-    } else {  // spare slots re-read chunk 0
-        coff8 = 0;
-        loff8 = b4::SPARE8 + 8 * (t - b4::N8);
-    }
-    if (t < b4::N4) {
-        const int rz = t / b4::RY, ry = t % b4::RY;
-        coff4 = (int)(rz * sp + (long)ry * g.nxp + 16);
-        loff4 = rz * b4::SLICE + ry * b4::ROW + 16;
-    } else {
-        coff4 = 0;
-        loff4 = b4::SPARE4 + 4 * (t - b4::N4);
-    }
+    const int cg = lane & 3;
+    // one wave also runs slot 8; which one rotates with the workgroup, so
+    // that the workgroups sharing a CU put that work on different SIMDs
+    const bool two = wave == (int)(blockIdx.x & 7);
+    auto slot_place = [&](int sl, int &coff, int &lrow) __attribute__((always_inline)) {
+        const int j = lane >> 2;
+        const int ry = 4 * (sl % 3) + ((j >> 1) & 1) + 2 * (j >> 3);
+        const int rz = 4 * (sl / 3) + ((j >> 2) & 1) + 2 * (__builtin_popcount(j & 7) & 1);
+        coff = (int)(rz * sp + (long)ry * g.nxp + 8 * cg);
+        lrow = rz * b4::SLICE + ry * b4::ROW + 16 * cg;
+    };
+    int coffA, lrowA, coffX = 0, lrowX = 0;
+    slot_place(wave, coffA, lrowA);
+    if (two) slot_place(8, coffX, lrowX);
+
     // this lane's block and outputs
-    const int xq = wave & 3, zh = wave >> 2;
-    const int n = lane & 15, yb = n & 3, zz = n >> 2;
-    const int xb = X0 + 4 * xq;
+    const int n = lane & 15, xb = n & 7, yb = n >> 3;
+    const int x0 = X0 + 4 * xb;
     const int y = Y0 + 4 * yb + q;
-    const int z = Z0 + 4 * zh + zz;
+    const int z = Z0 + wave;
     const TS *Rt = a.R + ((long)(Z0 + g.pz) * sp + (long)(Y0 + g.py) * g.nxp + (X0 + g.px));
-    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(y - Y0) * g.nxp + (xb - X0)) * sizeof(TS));
+    const unsigned rbyte = (unsigned)(((long)(z - Z0) * sp + (long)(y - Y0) * g.nxp + (x0 - X0)) * sizeof(TS));
     const TS *Rv = reinterpret_cast<const TS *>(reinterpret_cast<const char *>(Rt) + rbyte);
     const int zc = z < g.nz ? z : g.nz - 1;
     const float sig = a.sigma[zc];
     const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
-    // D operand: block n, K group q's row; MFMA o adds kOff[o], o = 4 its own
-    // per-group offset
-    const int bbase = (4 * zh + zz + 2) * b4::SLICE + (4 * yb + 2 + q) * b4::ROW + 4 * xq;
+    // D operand: block n's window xb (region columns 4 xb .. 4 xb + 7) of K
+    // group q's row; MFMA o adds kOff[o], o = 4 its own per-group offset
+    const int bbase = (wave + 2) * b4::SLICE + (4 * yb + 2 + q) * b4::ROW + 8 * xb;
     const int b4base = bbase + (q < 2 ? b4::SLICE + 3 * b4::ROW : -b4::SLICE + b4::ROW);
 
     float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1443,26 +1464,63 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     auto soff_of = [&](int lin) __attribute__((always_inline)) {
         return (unsigned)((lin - min_lin) * (int)sizeof(TS));
     };
-    chunk8 pn8;
-    chunk4 pn4;
+    // per slot: 8 columns, and columns 32-35 on lanes with cg = 3 (at 16
+    // bytes past their 8-column chunk)
+    chunk8 pnA, pnX;
+    chunk4 pqA = {}, pqX = {};
     auto load_chunks = [&](int lin) __attribute__((always_inline)) {
         const rsrc_t r = rsrc_p(lin);
         const unsigned so = soff_of(lin);
-        stg8.load(pn8, r, (unsigned)(coff8 * (int)sizeof(TS)), so);
-        stg4.load(pn4, r, (unsigned)(coff4 * (int)sizeof(TS)), so);
+        const unsigned oA = (unsigned)(coffA * (int)sizeof(TS)), oX = (unsigned)(coffX * (int)sizeof(TS));
+        stg8.load(pnA, r, oA, so);
+        if (cg == 3) stg4.load(pqA, r, oA + 8 * (unsigned)sizeof(TS), so);
+        if (two) {
+            stg8.load(pnX, r, oX, so);
+            if (cg == 3) stg4.load(pqX, r, oX + 8 * (unsigned)sizeof(TS), so);
+        }
     };
     // the chunks at the tile's own positions (the P(y) side of every
     // difference, search offset 0), scaled by -f, in registers
-    chunk8 p0r8;
-    chunk4 p0r4;
+    chunk8 p0A, p0X;
+    chunk4 q0A, q0X;
     load_chunks(0);
-    __builtin_memcpy(&p0r8, &pn8, sizeof(p0r8));
-    __builtin_memcpy(&p0r4, &pn4, sizeof(p0r4));
-    stg8.own_scaled(p0r8);
-    stg4.own_scaled(p0r4);
+    __builtin_memcpy(&p0A, &pnA, sizeof(p0A));
+    __builtin_memcpy(&q0A, &pqA, sizeof(q0A));
+    stg8.own_scaled(p0A);
+    stg4.own_scaled(q0A);
+    if (two) {
+        __builtin_memcpy(&p0X, &pnX, sizeof(p0X));
+        __builtin_memcpy(&q0X, &pqX, sizeof(q0X));
+        stg8.own_scaled(p0X);
+        stg4.own_scaled(q0X);
+    }
+    auto st16 = [](_Float16 *dst, const half2v *v) __attribute__((always_inline)) {
+        __builtin_memcpy(__builtin_assume_aligned(dst, 16), v, 16);
+    };
+    // window 2 cg (own 8 columns) and window 2 cg + 1 (upper four, then the
+    // next lane's lower four by DPP row shift, or columns 32-35 for cg = 3)
+    auto store_slot = [&](_Float16 *buf, const chunk8 &pn, const chunk8 &p0, const chunk4 &pq, const chunk4 &q0,
+                          int lrow) __attribute__((always_inline)) {
+        half2v d[4], e[4];
+        stg8.scaled(pn, p0, d);
+        e[0] = d[2];
+        e[1] = d[3];
+#pragma unroll
+        for (int h = 0; h < 2; h++)  // row_shl:1, lane i reads lane i + 1 of its 16-lane row
+            e[2 + h] = __builtin_bit_cast(half2v, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, d[h]), 0x101, 0xf, 0xf,
+                                                                            true));
+        if (cg == 3) {
+            half2v d4[2];
+            stg4.scaled(pq, q0, d4);
+            e[2] = d4[0];
+            e[3] = d4[1];
+        }
+        st16(buf + lrow, d);
+        st16(buf + lrow + 8, e);
+    };
     auto stage = [&](_Float16 *buf) __attribute__((always_inline)) {
-        stg8.store_scaled(pn8, p0r8, buf + loff8);
-        stg4.store_scaled(pn4, p0r4, buf + loff4);
+        store_slot(buf, pnA, p0A, pqA, q0A, lrowA);
+        if (two) store_slot(buf, pnX, p0X, pqX, q0X, lrowX);
     };
     struct RVals {
         TS v[4];
@@ -1513,13 +1571,13 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
         load_r(rout, l0);
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         constexpr int NR = b4::NOPS;
-        constexpr int KD = 2;
+        constexpr int KD = INLM_B4_KD;
         auto b_of = [&](auto oi) __attribute__((always_inline)) {
             constexpr int o = decltype(oi)::value;
             if constexpr (o == 4)
-                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + b4base, 8));
+                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + b4base, 16));
             else
-                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + bbase + b4::kOff[o], 8));
+                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + bbase + b4::kOff[o], 16));
         };
         half8 ab[KD];
         mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = b_of(ri); });
@@ -1561,7 +1619,7 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     if (y < g.ny && z < g.nz) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int x = xb + i;
+            const int x = x0 + i;
             if (x < g.nx) {
                 const long o = ((long)z * g.ny + y) * g.nx + x;
                 float res = a.raw[o];

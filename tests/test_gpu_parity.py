@@ -166,29 +166,6 @@ def test_division_selftest_exhaustive(gpu_available):
     assert mism.value == 0, f"{mism.value} mismatches, e.g. bits {ex.value:#x}"
 
 
-def test_cpp_facade_drop_in(gpu_available, tmp_path):
-    """The header-only C++ drop-in (include/iternlm_gpu.h), driven like the
-    reference's main.cpp, reproduces the reference fixture bit for bit."""
-    import os
-    import subprocess
-    from conftest import ROOT
-    exe = os.path.join(ROOT, "tests", "cpp", "facade_main")
-    assert os.path.exists(exe), "tests/cpp/facade_main is not built (run __graft_entry__.build())"
-    f = load_npz("filter_p222_s3_f32.npz")
-    nz, ny, nx = f["raw"].shape
-    f["raw"].astype(np.float32).tofile(tmp_path / "raw.bin")
-    f["sigma1"].astype(np.float32).tofile(tmp_path / "sigma1.bin")
-    f["sigma2"].astype(np.float32).tofile(tmp_path / "sigma2.bin")
-    print("[gpu] C++ facade", flush=True)
-    r = subprocess.run([exe, str(tmp_path), str(nx), str(ny), str(nz), str(int(f["rs"][0])), str(int(f["rp"][0])),
-                        "0"], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    out1 = np.fromfile(tmp_path / "out1.bin", np.float32).reshape(nz, ny, nx)
-    out2 = np.fromfile(tmp_path / "out2.bin", np.float32).reshape(nz, ny, nx)
-    assert np.array_equal(bits(out1), bits(f["out1"]))
-    assert np.array_equal(bits(out2), bits(f["out2"]))
-
-
 # Error budgets of the matrix-core forms against the FP64 oracle of the same
 # algorithm, in units of the phantom's white-noise sigma (one iteration, same
 # sigma list).  FP32_FAST differs by FP16 D planes and fused arithmetic; FP16
