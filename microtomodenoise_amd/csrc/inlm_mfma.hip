@@ -789,19 +789,27 @@ void filter_mfma(MfmaArgs<TS> a)
         }
     }
 
-    // FP8 planes hold D * S with S = 256 f^2, f = 2^m per tile, chosen so
-    // that the distance at the weight cutoff of the tile's middle slice
-    // (|a| = 3.57, d = 3.57 / |mm|) lands in [2, 8] after scaling: inside
-    // the FP8 normal range [2^-6, 448], and below what one clamped tap adds
-    // (256 times its weight; 10 for the smallest r = 2 weight, 0.04), so a
-    // patch pair with one saturated tap stays beyond the cutoff.  The band
-    // holds the weights times a.wscale, so the accumulator is
-    // d * S * wscale.
+    // FP8 planes hold D * S with S = 256 f^2, f = 2^m per tile.  One
+    // saturated tap (its square clamped to 1, times 256) adds at least
+    // 256 * (15/16) * wmin * wscale to the accumulator (an FP8 e4m3 code of
+    // a weight lies within 1/16 of it), so 4^m <= (15/32) wmin |mm| / 3.56648
+    // with |mm| the smallest over the tile's slices puts twice the largest
+    // cutoff distance of the tile below what that tap alone adds: a patch
+    // pair with a saturated tap keeps weight exactly zero at every patch
+    // radius and under z-varying sigma.  The cutoff then lands between 0.25
+    // and 1 times 120 wmin in plane units (2.4 at r = 3 and constant
+    // sigma), inside the FP8 normal range [2^-6, 448].  f is staged as a
+    // normal FP16 number, so 4^m stays within [4^-12, 4^8]; at the lower
+    // clamp (a noise level thousands of standard deviations of the
+    // standardised volume) a tap saturates only at a difference of 4096
+    // standard deviations.  The band holds the weights times a.wscale, so
+    // the accumulator is d * S * wscale.
     float f8f = 1.f;
     if constexpr (kF8) {
-        const int zm = Z0 + MTZ / 2 < g.nz ? Z0 + MTZ / 2 : g.nz - 1;
-        const float smid = a.sigma[zm];
-        const float x = __builtin_fabsf((-1.f / (smid * smid * a.beta)) * a.mscale) * (8.f / 3.56648f / 256.f);
+        float smax = 0.f;
+#pragma unroll
+        for (int zz = 0; zz < MTZ; zz++) smax = __builtin_fmaxf(smax, a.sigma[Z0 + zz < g.nz ? Z0 + zz : g.nz - 1]);
+        const float x = __builtin_fabsf((-1.f / (smax * smax * a.beta)) * a.mscale) * a.wmin * (15.f / 32.f / 3.56648f);
         // exponent of x from its bits (0 and subnormals give -127, infinity
         // and NaN 128; both clamp), S / 256 = 4^m <= x
         int e = (int)((__builtin_bit_cast(unsigned, x) >> 23) & 0xffu) - 127;

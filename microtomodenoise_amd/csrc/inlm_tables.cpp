@@ -98,6 +98,12 @@ bool cpu_weight_table(const int rp[3], int nslices, float z_aniso, int nz_padded
         out.assign(patch_offsets(rp, nslices).size(), 1.f / 7.f);
         return true;
     }
+    // The literal tables hold isotropic weights only.  With z anisotropy the
+    // reference GPU class applies the computed table at every radius but
+    // (1,1,3) (iternlm_gpu.cu:1470, :1700-1720: its generic kernel runs after,
+    // and overwrites, every specialised one except patch113's), so -aniso
+    // takes effect here as it does there.
+    if (kind == 1 && z_aniso != 1.f && !(a == 1 && b == 1 && c == 3)) kind = 0;
     if (kind == 0) {
         out = computed_weights(rp, nslices, z_aniso, nz_padded);
         return true;

@@ -114,16 +114,23 @@ def _raster_ellipsoid(lab, rng, c, axes, value):
     rot = np.array([[a * a + b * b - cc * cc - d * d, 2 * (b * cc - a * d), 2 * (b * d + a * cc)],
                     [2 * (b * cc + a * d), a * a - b * b + cc * cc - d * d, 2 * (cc * d - a * b)],
                     [2 * (b * d - a * cc), 2 * (cc * d + a * b), a * a - b * b - cc * cc + d * d]])
-    rmax = max(axes)
-    x0, x1 = max(int(c[0] - rmax), 0), min(int(c[0] + rmax) + 2, nx)
-    y0, y1 = max(int(c[1] - rmax), 0), min(int(c[1] + rmax) + 2, ny)
-    z0, z1 = max(int(c[2] - rmax), 0), min(int(c[2] + rmax) + 2, nz)
+    # inside: x^T M x <= 1, M = rot diag(1/a^2) rot^T (ellipsoid frame p = x rot);
+    # the half extent along axis j is sqrt((M^-1)_jj) = sqrt(sum_i rot_ji^2 a_i^2)
+    a2 = np.asarray(axes, np.float64) ** 2
+    M = rot @ np.diag(1.0 / a2) @ rot.T
+    ext = np.sqrt((rot * rot) @ a2)
+    x0, x1 = max(int(math.floor(c[0] - ext[0])), 0), min(int(math.ceil(c[0] + ext[0])) + 1, nx)
+    y0, y1 = max(int(math.floor(c[1] - ext[1])), 0), min(int(math.ceil(c[1] + ext[1])) + 1, ny)
+    z0, z1 = max(int(math.floor(c[2] - ext[2])), 0), min(int(math.ceil(c[2] + ext[2])) + 1, nz)
     if x0 >= x1 or y0 >= y1 or z0 >= z1:
         return
-    zz, yy, xx = np.meshgrid(np.arange(z0, z1) - c[2], np.arange(y0, y1) - c[1], np.arange(x0, x1) - c[0],
-                             indexing="ij")
-    p = np.stack([xx, yy, zz], -1) @ rot  # coordinates in the ellipsoid frame
-    m = (p[..., 0] / axes[0]) ** 2 + (p[..., 1] / axes[1]) ** 2 + (p[..., 2] / axes[2]) ** 2 <= 1.0
+    zz = (np.arange(z0, z1) - c[2])[:, None, None]
+    yy = (np.arange(y0, y1) - c[1])[None, :, None]
+    xx = (np.arange(x0, x1) - c[0])[None, None, :]
+    # q = M00 x^2 + x (2 M01 y + 2 M02 z) + (M11 y^2 + M22 z^2 + 2 M12 y z)
+    lin = 2.0 * (M[0, 1] * yy + M[0, 2] * zz)
+    const = M[1, 1] * yy * yy + M[2, 2] * zz * zz + 2.0 * M[1, 2] * yy * zz
+    m = (M[0, 0] * xx + lin) * xx + const <= 1.0
     lab[z0:z1, y0:y1, x0:x1][m] = value
 
 

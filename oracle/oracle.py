@@ -162,7 +162,9 @@ def literal_weights(w) -> np.ndarray:
 
 
 def cpu_weight_table(p: Params, nz_padded: int) -> np.ndarray:
-    """The weight table the reference CPU path applies for these parameters."""
+    """The weight table the reference CPU path applies for these parameters
+    (Denoiser/iternlm_cpu.cpp:97-114, :529-548); with z anisotropy the literal
+    radii but (1,1,3) take the computed table of the reference GPU class."""
     rp = p.radius_patchspace
     kind = orc().orc_cpu_weight_kind(_i32(rp), int(p.blockwise))
     if kind < 0:
@@ -170,8 +172,10 @@ def cpu_weight_table(p: Params, nz_padded: int) -> np.ndarray:
     wc = distweight(rp, p.nslices, p.z_anisotropy, nz_padded)
     if kind == 2:
         return np.full_like(wc, np.float32(1.0) / np.float32(7.0))
-    if kind == 1:
+    if kind == 1 and (p.z_anisotropy == 1.0 or tuple(rp) == (1, 1, 3)):
         return literal_weights(distweight(rp, p.nslices, 1.0, nz_padded))
+    # anisotropic voxels: the computed table, as the reference GPU class's
+    # generic kernel applies it (Denoiser/iternlm_gpu.cu:1470, :1700-1720)
     return wc
 
 
@@ -230,6 +234,48 @@ def filter_iteration(R: np.ndarray, P: np.ndarray, sigma, p: Params, *, precisio
         sg = np.ascontiguousarray(sigma, np.float64)
         orc().orc_filter_f64(Rp, Pp, shape, _i32(pad), sg, p.beta, len(soff), soff, len(poff), poff, w,
                              p.weight_mode, out, z0, z1, nthreads)
+    return out
+
+
+def reflect_index(n: int, pad: int) -> np.ndarray:
+    """Source index of every padded position along one axis of length n, from
+    the oracle's own reflect padding applied to the indices."""
+    idx = np.arange(n, dtype=np.float32).reshape(n, 1, 1)
+    return pad_reflect(idx, (0, 0, pad))[:, 0, 0].astype(np.int64)
+
+
+def filter_window(R: np.ndarray, P: np.ndarray, sigma, p: Params, z_range, y_range, *, precision="f32",
+                  weights=None, nthreads=0) -> np.ndarray:
+    """filter_iteration restricted to the output window z in [z0, z1), y in
+    [y0, y1), all x, on a volume too large to pad whole: the window and its
+    halo are gathered through the reflect padding's index maps, so every
+    output value equals filter_iteration's at that voxel.  Returns the window
+    (z1 - z0, y1 - y0, nx)."""
+    nz, ny, nx = R.shape
+    z0, z1 = z_range
+    y0, y1 = y_range
+    if not (0 <= z0 < z1 <= nz and 0 <= y0 < y1 <= ny):
+        raise ValueError("window outside the volume")
+    px, py, pz = p.padding()
+    zi = reflect_index(nz, pz)[z0:z1 + 2 * pz]
+    yi = reflect_index(ny, py)[y0:y1 + 2 * py]
+    xi = reflect_index(nx, px)
+    ix = np.ix_(zi, yi, xi)
+    Rw = np.ascontiguousarray(np.asarray(R, np.float32)[ix])
+    Pw = np.ascontiguousarray(np.asarray(P, np.float32)[ix])
+    nxp, nyp = nx + 2 * px, (y1 - y0) + 2 * py
+    S = search_offsets(p.radius_searchspace, p.nslices)
+    PT = patch_offsets(p.radius_patchspace, p.nslices)
+    soff, poff = linear(S, nxp, nyp), linear(PT, nxp, nyp)
+    # the table of the whole padded volume (the r=1 rule depends on its depth)
+    w = cpu_weight_table(p, nz + 2 * pz) if weights is None else np.ascontiguousarray(weights, np.float32)
+    shape = _i32([nx, y1 - y0, z1 - z0])
+    dt = np.float32 if precision == "f32" else np.float64
+    sg = np.ascontiguousarray(np.asarray(sigma)[z0:z1], dt)
+    out = np.zeros((z1 - z0, y1 - y0, nx), dt)
+    fn = orc().orc_filter_f32 if precision == "f32" else orc().orc_filter_f64
+    fn(Rw, Pw, shape, _i32((px, py, pz)), sg, p.beta, len(soff), soff, len(poff), poff, w, p.weight_mode, out,
+       0, z1 - z0, nthreads)
     return out
 
 

@@ -94,8 +94,32 @@ def test_product_tables_match_reference_fixtures():
         assert np.array_equal(O.linear(S, shp[0], shp[1]), z[f"c{i}_soff"]), i
         assert np.array_equal(O.linear(PT, shp[0], shp[1]), z[f"c{i}_poff"]), i
         kind = O.orc().orc_cpu_weight_kind(np.asarray(rp, np.int32), 1)
-        if kind == 0:
+        # the fixture holds the reference's setup_distweight table: the
+        # product applies it for computed radii and, with z anisotropy, for
+        # the literal radii too (the reference GPU class's generic kernel)
+        if kind == 0 or (kind == 1 and p.z_anisotropy != 1.0):
             assert np.array_equal(w.view(np.uint32), z[f"c{i}_w"].view(np.uint32)), i
+
+
+def test_anisotropy_reaches_the_literal_radii():
+    # rp (2,2,2) is a literal-table radius of the CPU kernels; z anisotropy
+    # must still change its weights (only z taps lose weight), while (1,1,3)
+    # keeps its literal table as the reference GPU class's patch113 kernel does
+    def table(rp, aniso):
+        p = _capi.default_params()
+        p.radius_searchspace[:] = [3, 3, 3]
+        p.radius_patchspace[:] = list(rp)
+        p.z_anisotropy = aniso
+        return _capi.tables(p, [30, 30, 30])
+    _, PT, w1 = table((2, 2, 2), 1.0)
+    _, _, w2 = table((2, 2, 2), 2.0)
+    dz = PT[:, 2] != 0
+    assert np.all(w2[dz] < w1[dz]) and np.allclose(w2[~dz], w1[~dz], rtol=1e-6)
+    op = O.Params(radius_searchspace=(3, 3, 3), radius_patchspace=(2, 2, 2), z_anisotropy=2.0)
+    assert np.array_equal(w2, O.cpu_weight_table(op, 40))
+    _, _, a = table((1, 1, 3), 1.0)
+    _, _, b = table((1, 1, 3), 2.0)
+    assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("rp,blockwise", [((1, 1, 1), 1), ((2, 2, 2), 1), ((3, 3, 3), 0), ((3, 3, 3), 1),

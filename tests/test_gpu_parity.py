@@ -201,15 +201,21 @@ def test_fast_forms_vs_fp64_oracle(gpu_available, form, rp, rs, nx):
     assert mx <= bmax and rms <= brms, (mx, rms)
 
 
+@pytest.mark.parametrize("rp", [(2, 2, 2), (3, 3, 3)])
 @pytest.mark.parametrize("form", ["fp32_fast", "fp16", "fp16_fp8"])
-def test_fast_forms_anisotropic_patch(gpu_available, form):
+def test_fast_forms_anisotropic_patch(gpu_available, form, rp):
     # z anisotropy makes the weights depend on (dx^2 + dy^2, dz^2) rather
     # than on the distance alone: the per-pattern (not per-class) operands
     spec = ph.PhantomSpec((40, 30, 20), seed=52, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8)
     raw = ph.make(spec)
     nz, ny, nx = raw.shape
-    rs, rp = (3, 3, 2), (2, 2, 2)
+    rs = (3, 3, 2)
     sig = np.full(nz, 700.0, np.float32)
+    nzp = nz + 2 * (min(11 // 2, rs[2]) + rp[2])
+    iso = O.cpu_weight_table(O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False), nzp)
+    ani = O.cpu_weight_table(O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False,
+                                      z_anisotropy=2.0), nzp)
+    assert not np.allclose(iso, ani, rtol=1e-3)  # the weights really depend on it
     print(f"[gpu] anisotropic patch, fast form {form}", flush=True)
     g, _ = make_gpu((nx, ny, nz), rs, rp, precision=form, z_anisotropy=2.0)
     g.set_sigma(sig)

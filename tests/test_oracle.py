@@ -245,3 +245,19 @@ def test_zwindow_includes_one_sample_past_window():
     assert out.shape == (6, 3)
     m_in, _ = O.create2clusters(vals, 2, 6, precision="f64")
     assert out[1, 1] == pytest.approx(m_in)
+
+
+@pytest.mark.parametrize("rp,rs", [((2, 2, 2), (3, 3, 2)), ((1, 1, 1), (2, 2, 2)), ((3, 3, 3), (2, 2, 1))])
+def test_filter_window_equals_whole_volume(rp, rs):
+    # the window form gathers its halo through the reflect maps: at the volume
+    # faces (the fold) and inside it must reproduce the whole-volume pass
+    rng = np.random.default_rng(11)
+    R = rng.normal(100.0, 10.0, (9, 17, 13)).astype(np.float32)
+    P = (R + rng.normal(0.0, 2.0, R.shape)).astype(np.float32)
+    sig = np.linspace(8.0, 12.0, 9).astype(np.float32)
+    p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False)
+    for prec in ("f32", "f64"):
+        whole = O.filter_iteration(R, P, sig, p, precision=prec)
+        for zr, yr in [((0, 2), (0, 5)), ((3, 6), (6, 11)), ((7, 9), (12, 17)), ((0, 9), (0, 17))]:
+            win = O.filter_window(R, P, sig, p, zr, yr, precision=prec)
+            assert np.array_equal(win, whole[zr[0]:zr[1], yr[0]:yr[1]]), (prec, zr, yr)
