@@ -23,8 +23,12 @@ metric's precision rule.
 Rooflines come from HIP events the library records on its own stream around
 each filter launch: algorithmic FLOPs per voxel-iteration F = N_s*(2*N_p+8)
 (difference and square per offset, a multiply and an add per tap, scale,
-weight, two accumulations, max) against the 157.3 TFLOP/s FP32 peak, and the
-unit that limits the kernel from the rocprofv3 counters summarised in
+weight, two accumulations, max) against the dense peak of the unit the
+kernel computes on (2.5 PFLOP/s FP16 MFMA for the 16-bit forms, 5.0 for the
+FP8-plane form, 157.3 TFLOP/s FP32 vector for the plain kernel, whose
+ceiling under the parity rules -- no packed FP32, no fused multiply-add -- is
+78.6), HBM traffic from the FETCH_SIZE/WRITE_SIZE counters, and the unit that
+limits the kernel from the rocprofv3 counters summarised in
 profiles/counters_<workload>.json.
 
 The CPU baseline is the reference's own CPU kernel (oracle/_ref, compiled
@@ -47,6 +51,8 @@ FP32_PEAK_TFLOPS = 157.3
 # per instruction at two VALU instructions per SIMD per four cycles
 FP32_PARITY_CEILING_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# dense matrix-core peaks of the unit each form's distance sums run on
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp32_fast": 2500.0, "fp16_fp8": 5000.0}
 PSNR_BUDGET_DB = 0.05
 CONTRACT_FORMS = ("bf16", "fp16")
 
@@ -415,19 +421,27 @@ def main():
                   "faster_than_unpruned_within_budget": bool(best and best["value"] > accel[head]["value"])}
     del clean_t
 
-    def roof(kernel, kms, plain_kernel=False):
+    def roof(kernel, kms, form=None):
+        """Roofline of one filter kernel: algorithmic FLOPs F per voxel-
+        iteration over its mean launch time, against the peak of the unit it
+        computes on (dense FP16 or FP8 MFMA for the matrix-core forms, FP32
+        vector for the plain kernel), plus what the counters say limits it."""
         km = float(np.mean(kms))
         ach = F * V / (km * 1e-3) / 1e12
         c = counters_of(args.workload, kernel)
         traffic = c.get("traffic_bytes") if c else None
-        hbm_s = traffic / (HBM_PEAK_GBS * 1e9) if traffic else None
-        out = {"bound": "hbm" if hbm_s and hbm_s > 0.5 * km * 1e-3 else "mfma", "kernel": kernel,
-               "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-               "frac": round(ach / FP32_PEAK_TFLOPS, 4), "flops_per_voxel_iter": F, "kernel_ms": round(km, 3),
-               "traffic": traffic}
-        if plain_kernel:
-            out["frac_of_parity_ceiling"] = round(ach / FP32_PARITY_CEILING_TFLOPS, 4)
-            out["parity_ceiling_tflops"] = FP32_PARITY_CEILING_TFLOPS
+        if form is None:
+            out = {"bound": "valu", "kernel": kernel, "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
+                   "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
+                   "frac_of_parity_ceiling": round(ach / FP32_PARITY_CEILING_TFLOPS, 4),
+                   "parity_ceiling_tflops": FP32_PARITY_CEILING_TFLOPS}
+        else:
+            peak = MFMA_PEAK_TFLOPS[form]
+            out = {"bound": "mfma", "kernel": kernel, "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                   "frac": round(ach / peak, 4), "frac_of_fp32_vector_peak": round(ach / FP32_PEAK_TFLOPS, 4)}
+        out.update({"flops_per_voxel_iter": F, "kernel_ms": round(km, 3), "traffic": traffic})
+        if traffic:
+            out["hbm_frac"] = round(traffic / (km * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
         if c:
             out["limiter"] = c.get("limiter")
             out["counters"] = {k: v for k, v in c.items() if k not in ("limiter", "traffic_bytes")}
@@ -446,14 +460,14 @@ def main():
 
     if head is not None:
         best = (f"{head}", accel[head]["value"], accel[head]["ms_per_step"], mfma_kernel(head),
-                runs[head]["kms"], DTYPES[head])
+                runs[head]["kms"], DTYPES[head], head)
         if pruned and pruned["fastest_within_budget"] and pruned["faster_than_unpruned_within_budget"]:
             p = pruned["fastest_within_budget"]
             best = (f"{head} + learned pruning (K={p['K']}, threshold {p['threshold']})", p["value"],
-                    p["ms_per_step"], mfma_kernel(head), [p["kernel_ms"]], DTYPES[head])
+                    p["ms_per_step"], mfma_kernel(head), [p["kernel_ms"]], DTYPES[head], head)
     else:
         best = ("fp32 parity (no 16-bit form within the budget)", v_plain, plain["wall"] / args.steps * 1e3,
-                plain_kernel, plain["kms"], "f32")
+                plain_kernel, plain["kms"], "f32", None)
     line = {
         "metric": METRIC,
         "value": round(best[1], 3),
@@ -479,8 +493,8 @@ def main():
                    "accelerated": accel.get(head) if head else None,
                    "other_forms": [accel[f] for f in forms if f != head],
                    "learned_pruning": pruned},
-        "roofline": roof(best[3], best[4]),
-        "roofline_plain_fp32": roof(plain_kernel, plain["kms"], plain_kernel=True),
+        "roofline": roof(best[3], best[4], best[6]),
+        "roofline_plain_fp32": roof(plain_kernel, plain["kms"]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

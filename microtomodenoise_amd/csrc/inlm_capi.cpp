@@ -249,7 +249,7 @@ size_t handle_bytes(const inlm_params &pin, const int32_t shape_in[3])
     const size_t padb = (shifted > 0 ? kGuard + shifted + pad_elems : pad_elems + kGuard) * esz;
     const size_t vox = (size_t)g.voxels();
     size_t b = 2 * padb + sizeof(float) * vox + sizeof(float) * (size_t)g.nz;
-    if (fast) b += sizeof(float) * vox + 2 * sizeof(double);
+    if (fast) b += sizeof(float) * vox + inlm::kMomentDoubles * sizeof(double);
     const size_t ns = search_offsets(p.radius_searchspace, p.nslices).size();
     b += 2 * ns * sizeof(SearchEntry);
     if (p.noise_mode != INLM_NOISE_MANUAL && p.n_samples > 0) b += (size_t)p.n_samples * (sizeof(int64_t) + sizeof(double));
@@ -418,7 +418,7 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
     if (h->fast) {
         if ((e = hipMalloc(&h->raw, sizeof(float) * g.voxels())) != hipSuccess)
             return bail(hip_fail(nullptr, e, "hipMalloc(raw)"));
-        if ((e = hipMalloc(&h->moments, 2 * sizeof(double))) != hipSuccess)
+        if ((e = hipMalloc(&h->moments, inlm::kMomentDoubles * sizeof(double))) != hipSuccess)
             return bail(hip_fail(nullptr, e, "hipMalloc(moments)"));
     } else {
         h->R_pad = static_cast<float *>(h->Rs);
@@ -509,7 +509,6 @@ inlm_status inlm_load_raw(inlm_handle *h, const float *raw, int32_t is_device)
         // normalisation: shift = mean, scale = 1/std of the original (FP64)
         HIP_TRY(h, hipMemcpyAsync(h->raw, raw, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                   h->stream));
-        HIP_TRY(h, hipMemsetAsync(h->moments, 0, 2 * sizeof(double), h->stream));
         HIP_TRY(h, inlm::launch_moments(h->raw, (long)h->geo.voxels(), h->moments, h->stream));
         double mom[2];
         HIP_TRY(h, hipMemcpyAsync(mom, h->moments, sizeof mom, hipMemcpyDeviceToHost, h->stream));

@@ -96,6 +96,11 @@ hipError_t launch_filter_mfma(int form, int rp, int weight_fn, const void *Rs, c
                               bool row_form = false);
 hipError_t launch_pad_normalise(int form, const float *in, void *out, long shifted, const FilterGeom &g, float shift,
                                 float scale, hipStream_t st);
+// FP64 sum and sum of squares of n floats into acc[0], acc[1]; acc holds
+// kMomentDoubles doubles (the rest are per-workgroup partials), needs no
+// clearing, and the result does not depend on scheduling.
+constexpr int kMomentBlocks = 1024;
+constexpr int kMomentDoubles = 2 + 2 * kMomentBlocks;
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);
 inline size_t form_elem_size(int form) { return form == FORM_FP32 || form == FORM_FP32_FAST ? 4 : 2; }
 

@@ -593,7 +593,7 @@ hipError_t ProposalNet::init(int ns, int device, uint64_t seed)
     if ((e = hipMalloc(&d->m1, bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d->m2, bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d->loss, sizeof(float))) != hipSuccess) return e;
-    if ((e = hipMalloc(&d->moments, 2 * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&d->moments, kMomentDoubles * sizeof(double))) != hipSuccess) return e;
     for (int l = 0; l < 3; l++) {
         if ((e = hipMalloc(&d->wf[l], NTAP * C * C * sizeof(__bf16))) != hipSuccess) return e;
         if ((e = hipMalloc(&d->wb[l], NTAP * C * C * sizeof(__bf16))) != hipSuccess) return e;
@@ -670,7 +670,6 @@ hipError_t ProposalNet::run(const float *vol, const int shape[3], const float *m
     const unsigned cb = (unsigned)((nwave + 3) / 4);
 
     // input normalisation: (v - mean) / std of this volume (FP64 moments)
-    if ((e = hipMemsetAsync(m.moments, 0, 2 * sizeof(double), m.st)) != hipSuccess) return e;
     if ((e = launch_moments(vol, nv, m.moments, m.st)) != hipSuccess) return e;
     double mom[2];
     if ((e = hipMemcpyAsync(mom, m.moments, sizeof mom, hipMemcpyDeviceToHost, m.st)) != hipSuccess) return e;

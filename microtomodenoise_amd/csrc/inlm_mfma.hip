@@ -1666,8 +1666,10 @@ __global__ __launch_bounds__(256) void pad_normalise_kernel(const float *__restr
     if (shifted > 0) out[shifted + idx - 1] = o;  // the copy shifted by one element
 }
 
-// Sum and sum of squares of a float volume in FP64 (one atomic pair per
-// workgroup).
+// Sum and sum of squares of a float volume in FP64, deterministic: each of
+// kMomentBlocks workgroups writes its partial pair to acc[2 + 2 b], and one
+// workgroup then sums the partials in a fixed order into acc[0], acc[1], so
+// the normalisation (and every 16-bit form's output) repeats bit for bit.
 __global__ __launch_bounds__(256) void moments_kernel(const float *__restrict__ in, long n, double *acc)
 {
     double s = 0.0, s2 = 0.0;
@@ -1689,8 +1691,34 @@ __global__ __launch_bounds__(256) void moments_kernel(const float *__restrict__ 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
-        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+        acc[2 + 2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        acc[3 + 2 * blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void moments_final_kernel(double *acc)
+{
+    // thread t sums partials t, t + 256, ... in order; then a fixed tree
+    double s = 0.0, s2 = 0.0;
+    for (int b = threadIdx.x; b < kMomentBlocks; b += 256) {
+        s += acc[2 + 2 * b];
+        s2 += acc[3 + 2 * b];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    __shared__ double red[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = s;
+        red[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        acc[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        acc[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
@@ -1997,7 +2025,10 @@ hipError_t launch_pad_normalise(int form, const float *in, void *out, long shift
 
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st)
 {
-    hipLaunchKernelGGL(moments_kernel, dim3(1024), dim3(256), 0, st, in, n, acc);
+    hipLaunchKernelGGL(moments_kernel, dim3(kMomentBlocks), dim3(256), 0, st, in, n, acc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, st, acc);
     return hipGetLastError();
 }
 

@@ -259,3 +259,23 @@ def test_block_kernel_equals_row_kernel(gpu_available, form, shape, rs):
     mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
     print(f"[gpu]   block vs row: max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
     assert mx < 0.02 and rms < 2e-4, (mx, rms)
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16", "fp16_fp8"])
+def test_fast_forms_repeat_bit_for_bit(gpu_available, form):
+    # the 16-bit forms normalise by the volume's FP64 moments; their
+    # reduction runs in a fixed order, so two handles on the same volume
+    # give the same result bit for bit
+    spec = ph.PhantomSpec((200, 96, 40), seed=57, radii=(2, 9), noise_sigma=1500.0, blur_sigma=0.8)
+    raw = ph.make(spec)
+    nz, ny, nx = raw.shape
+    sig = np.full(nz, 900.0, np.float32)
+    print(f"[gpu] repeatability of {form}", flush=True)
+    outs = []
+    for _ in range(2):
+        g, _ = make_gpu((nx, ny, nz), (3, 3, 3), (2, 2, 2), precision=form)
+        g.set_sigma(sig)
+        g.Run_GaussianNoise(1, raw)
+        outs.append(g.get_result())
+        g.free_device()
+    assert np.array_equal(bits(outs[0]), bits(outs[1]))
