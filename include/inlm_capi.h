@@ -182,6 +182,23 @@ INLM_API inlm_status inlm_draw_samples_mask(const int32_t shape[3], int64_t n, i
  * result. */
 INLM_API inlm_status inlm_patchstd(inlm_handle *h, int32_t which, const int64_t *samples, int64_t n, double *out);
 
+/* No counterpart in the reference.  inlm_patchstd for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): no handle, no hipMalloc/hipFree.  `vol`
+ * is a device pointer to an unpadded float volume of `shape` (x fastest);
+ * `samples` (host, n patch corners as for inlm_patchstd) and `out` (host, n
+ * doubles) are the caller's.  The sample list and the per-sample results
+ * are staged in a fixed device array of this library of
+ * INLM_NOALLOC_MAX_SAMPLES * 16 bytes (4 MiB), which each GPU the call runs
+ * on holds until the process ends; larger n is rejected with INLM_BAD_ARG.
+ * Every call on one device shares that array (each call overwrites it, so
+ * nothing needs clearing): calls that overlap on one GPU race.  Runs on the
+ * null stream and returns when `out` is written.  Same kernel, same values
+ * as inlm_patchstd. */
+#define INLM_NOALLOC_MAX_SAMPLES 262144
+INLM_API inlm_status inlm_patchstd_noalloc(const float *vol, const int32_t shape[3], int32_t patchsize,
+                                           const int64_t *samples, int64_t n, double *out, int32_t device);
+
 /* Noise level per slice (NoiseLevel::get_noiselevel, noiselevel.cpp:9-95):
  * patch std on the GPU, two-cluster EM in FP64 on the host, global or
  * z-adaptive, sigma = mean + stds_from_mean * std.  Writes nz floats. */

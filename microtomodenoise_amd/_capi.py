@@ -39,7 +39,10 @@ EXPORTS = [
     "inlm_rgb_tiff_shape", "inlm_rgb_tiff_read", "inlm_rgb_tiff_write", "inlm_denoise_rgb_files",
     # noise estimator host half, circular sampling mask, slab drivers
     "inlm_draw_samples_mask", "inlm_sigma_from_patchstd", "inlm_capacity_slices", "inlm_iterate_slabs",
+    # patch standard deviations without device allocation
+    "inlm_patchstd_noalloc",
 ]
+NOALLOC_MAX_SAMPLES = 262144  # INLM_NOALLOC_MAX_SAMPLES
 
 
 class InlmRgbOptions(C.Structure):
@@ -123,6 +126,7 @@ def lib():
         L.inlm_get_result.argtypes = [H, C.c_void_p, C.c_int32]
         L.inlm_draw_samples.argtypes = [i32p, C.c_int64, C.c_int32, C.c_uint64, i64p]
         L.inlm_patchstd.argtypes = [H, C.c_int32, i64p, C.c_int64, f64p]
+        L.inlm_patchstd_noalloc.argtypes = [C.c_void_p, i32p, C.c_int32, i64p, C.c_int64, f64p, C.c_int32]
         L.inlm_estimate_noise.argtypes = [H, C.c_int32, i64p, C.c_int64, f32p]
         L.inlm_denoise.argtypes = [H, f32p, f32p, f32p]
         L.inlm_last_filter_ms.argtypes = [H]
@@ -200,6 +204,18 @@ def default_params() -> InlmParams:
     p = InlmParams()
     lib().inlm_default_params(C.byref(p))
     return p
+
+
+def patchstd_noalloc(vol_dev_ptr: int, shape_xyz, patchsize: int, samples, device: int = 0) -> np.ndarray:
+    """Patch standard deviations of a device volume (unpadded float32, x
+    fastest) at the sample corners, through the entry point that allocates
+    no device memory (at most NOALLOC_MAX_SAMPLES samples per call)."""
+    s = np.ascontiguousarray(samples, np.int64)
+    out = np.zeros(len(s), np.float64)
+    sh = np.asarray(shape_xyz, np.int32)
+    check(lib().inlm_patchstd_noalloc(C.c_void_p(vol_dev_ptr), ptr(sh, C.c_int32), patchsize, ptr(s, C.c_int64),
+                                      len(s), ptr(out, C.c_double), device))
+    return out
 
 
 def draw_samples(shape_xyz, n: int, patchsize: int = 15, seed: int = 0, mask_diameter: int = 0) -> np.ndarray:

@@ -16,8 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinlm.so")
 
-SOURCES = ["inlm_kernels.hip", "inlm_mfma.hip", "inlm_learn.hip", "inlm_capi.cpp", "inlm_tables.cpp", "inlm_io.cpp",
-           "inlm_slabs.cpp", "inlm_rgb.cpp"]
+SOURCES = ["inlm_kernels.hip", "inlm_mfma.hip", "inlm_learn.hip", "inlm_noalloc.hip", "inlm_capi.cpp",
+           "inlm_tables.cpp", "inlm_io.cpp", "inlm_slabs.cpp", "inlm_rgb.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
