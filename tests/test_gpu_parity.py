@@ -172,7 +172,13 @@ def test_division_selftest_exhaustive(gpu_available):
 # and BF16 also store the volumes in 16 bits (BF16 keeps 8 significant bits,
 # so its budget is the loosest); FP16_FP8 keeps FP16 storage and rounds the
 # squared differences and the patch weights to FP8 (4 significant bits).
-FAST_BUDGET = {"fp32_fast": (0.05, 0.004), "fp16": (0.08, 0.006), "bf16": (1.0, 0.06), "fp16_fp8": (1.0, 0.02)}
+# The largest single errors come from patch pairs at the weight's cutoff,
+# where the reference's Pade weight jumps by 0.025 of the maximum: rounding
+# moves a distance across it and switches one term of the sum; hence the
+# fraction of voxels beyond 0.1 sigma next to max and RMS.
+FAST_BUDGET = {"fp32_fast": (0.01, 2e-4), "fp16": (0.12, 0.002), "bf16": (0.8, 0.008), "fp16_fp8": (0.8, 0.01)}
+# largest fraction of voxels further than 0.1 noise sigma from the oracle
+FAST_FRAC = {"fp32_fast": 0.0, "fp16": 0.0, "bf16": 0.001, "fp16_fp8": 0.001}
 
 
 @pytest.mark.parametrize("form", ["fp32_fast", "fp16", "bf16", "fp16_fp8"])
@@ -195,10 +201,10 @@ def test_fast_forms_vs_fp64_oracle(gpu_available, form, rp, rs, nx):
     p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False)
     ref = O.filter_iteration(raw, raw, sig.astype(np.float64), p, precision="f64")
     err = np.abs(got - ref) / spec.noise_sigma
-    mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
-    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
+    mx, rms, frac = float(err.max()), float(np.sqrt(np.mean(err ** 2))), float(np.mean(err > 0.1))
+    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units), {frac:.4f} of voxels beyond 0.1", flush=True)
     bmax, brms = FAST_BUDGET[form]
-    assert mx <= bmax and rms <= brms, (mx, rms)
+    assert mx <= bmax and rms <= brms and frac <= FAST_FRAC[form], (mx, rms, frac)
 
 
 @pytest.mark.parametrize("rp", [(2, 2, 2), (3, 3, 3)])
@@ -225,10 +231,37 @@ def test_fast_forms_anisotropic_patch(gpu_available, form, rp):
     p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False, z_anisotropy=2.0)
     ref = O.filter_iteration(raw, raw, sig.astype(np.float64), p, precision="f64")
     err = np.abs(got - ref) / spec.noise_sigma
-    mx, rms = float(err.max()), float(np.sqrt(np.mean(err ** 2)))
-    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units)", flush=True)
+    mx, rms, frac = float(err.max()), float(np.sqrt(np.mean(err ** 2))), float(np.mean(err > 0.1))
+    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units), {frac:.4f} of voxels beyond 0.1", flush=True)
     bmax, brms = FAST_BUDGET[form]
-    assert mx <= bmax and rms <= brms, (mx, rms)
+    assert mx <= bmax and rms <= brms and frac <= FAST_FRAC[form], (mx, rms, frac)
+
+
+@pytest.mark.parametrize("rp", [(2, 2, 2), (3, 3, 3)])
+@pytest.mark.parametrize("form", ["fp16", "bf16", "fp16_fp8"])
+def test_fast_forms_zvarying_sigma(gpu_available, form, rp):
+    # sigma rising threefold along z: the per-tile plane scales take the
+    # tile's largest sigma, so slices with small sigma keep their cutoff
+    # (a saturated tap still gives weight zero there)
+    spec = ph.PhantomSpec((44, 30, 24), seed=53, radii=(2, 7), noise_sigma=1500.0, blur_sigma=0.8, streaks=3,
+                          streak_len=(5, 15))
+    raw = ph.make(spec)
+    nz, ny, nx = raw.shape
+    rs = (3, 3, 2)
+    sig = np.linspace(400.0, 1200.0, nz).astype(np.float32)
+    print(f"[gpu] z-varying sigma, fast form {form} rp={rp}", flush=True)
+    g, _ = make_gpu((nx, ny, nz), rs, rp, precision=form)
+    g.set_sigma(sig)
+    g.Run_GaussianNoise(1, raw)
+    got = g.get_result().astype(np.float64)
+    g.free_device()
+    p = O.Params(radius_searchspace=rs, radius_patchspace=rp, blockwise=False)
+    ref = O.filter_iteration(raw, raw, sig.astype(np.float64), p, precision="f64")
+    err = np.abs(got - ref) / spec.noise_sigma
+    mx, rms, frac = float(err.max()), float(np.sqrt(np.mean(err ** 2))), float(np.mean(err > 0.1))
+    print(f"[gpu]   max {mx:.3e} rms {rms:.3e} (noise-sigma units), {frac:.4f} of voxels beyond 0.1", flush=True)
+    bmax, brms = FAST_BUDGET[form]
+    assert mx <= bmax and rms <= brms and frac <= FAST_FRAC[form], (mx, rms, frac)
 
 
 @pytest.mark.parametrize("form", ["fp16", "bf16"])
