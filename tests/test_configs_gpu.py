@@ -278,9 +278,9 @@ def test_cfg3_precision_forms_p3_s7_zadaptive(gpu_available):
 @pytest.mark.timeout(400)
 def test_cfg5_large_volume_p2_s7_six_iterations(gpu_available):
     """2048x2048x256 uint16 phantom with four grey levels and 1000 streaks,
-    patch r=2, search r=7, 6 iterations, alpha 0.6, z-adaptive noise; FP32
-    and BF16 (the configuration's form)."""
-    full_config("cfg5", 7, 2, "z-adaptive", 0.6, 6, ("bf16",), nz_win=16)
+    patch r=2, search r=7, 6 iterations, alpha 0.6, z-adaptive noise; FP32,
+    BF16 (the configuration's form) and FP16."""
+    full_config("cfg5", 7, 2, "z-adaptive", 0.6, 6, ("bf16", "fp16"), nz_win=16)
 
 
 @pytest.mark.timeout(400)
