@@ -205,7 +205,11 @@ def psnr_of(out, clean_t, peak):
 
 # 16-bit forms against the FP64 oracle, per pass, in noise-level units
 # (max, RMS, fraction of voxels beyond 0.1 noise level)
-BUDGET16 = {"fp16": (0.08, 0.006, 0.001), "bf16": (0.5, 0.06, 0.2)}
+# (the budgets of the small-volume form tests; the largest single errors are
+# patch pairs at the weight's cutoff, where the reference's weight jumps by
+# 0.025 of its maximum: one switched term moves a voxel by at most 0.025
+# times the local contrast)
+BUDGET16 = {"fp16": (0.35, 0.002, 0.001), "bf16": (0.8, 0.008, 0.001)}
 
 
 def check_16bit(form, vol, raw, spec, rs, rp, sigs, iters, clean_t, peak, plain_psnr, label, nz_win=16):
@@ -265,6 +269,14 @@ def test_cfg2_512_cube_p2_s5(gpu_available):
     """512^3 uint16 phantom with streaks, patch r=2, search r=5, 4 iterations,
     alpha 0.5, global noise; FP32 and the FP16 form."""
     full_config("cfg2", 5, 2, "global", 0.5, 4, ("fp16",), nz_win=16)
+
+
+@pytest.mark.timeout(400)
+def test_headline_p2_s7(gpu_available):
+    """The benchmark's workload: 1024x1024x512 float32 phantom (spheres, 200
+    streaks), patch r=2, search r=7, 4 iterations, alpha 0.5, global noise;
+    FP32, FP16 and BF16."""
+    full_config("headline", 7, 2, "global", 0.5, 4, ("fp16", "bf16"), nz_win=16)
 
 
 @pytest.mark.timeout(400)

@@ -174,11 +174,13 @@ def test_division_selftest_exhaustive(gpu_available):
 # squared differences and the patch weights to FP8 (4 significant bits).
 # The largest single errors come from patch pairs at the weight's cutoff,
 # where the reference's Pade weight jumps by 0.025 of the maximum: rounding
-# moves a distance across it and switches one term of the sum; hence the
-# fraction of voxels beyond 0.1 sigma next to max and RMS.
-FAST_BUDGET = {"fp32_fast": (0.01, 2e-4), "fp16": (0.12, 0.002), "bf16": (0.8, 0.008), "fp16_fp8": (0.8, 0.01)}
+# moves a distance across it and switches one term of the sum, which moves
+# the output by at most 0.025 |R - out| / sum(w) <= 0.025 * contrast (about
+# 0.33 noise sigma for these phantoms, 20000 between levels at sigma 1500).
+# Hence the fraction of voxels beyond 0.1 sigma next to max and RMS.
+FAST_BUDGET = {"fp32_fast": (0.35, 2e-4), "fp16": (0.35, 0.002), "bf16": (0.8, 0.008), "fp16_fp8": (0.8, 0.01)}
 # largest fraction of voxels further than 0.1 noise sigma from the oracle
-FAST_FRAC = {"fp32_fast": 0.0, "fp16": 0.0, "bf16": 0.001, "fp16_fp8": 0.001}
+FAST_FRAC = {"fp32_fast": 0.001, "fp16": 0.001, "bf16": 0.001, "fp16_fp8": 0.001}
 
 
 @pytest.mark.parametrize("form", ["fp32_fast", "fp16", "bf16", "fp16_fp8"])
