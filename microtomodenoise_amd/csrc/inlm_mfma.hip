@@ -1276,45 +1276,51 @@ void filter_mfma(MfmaArgs<TS> a)
 // unused for a 5-tap row and 14 for a 3-tap row; the seven operand reads
 // here replace eleven.
 //
-// A workgroup of 8 wavefronts owns a 32x8x8 output tile (the row form's
-// tile) and stages D over the tile and its patch halo, 36x12x12, in LDS
-// (double buffered, one barrier per offset).  Wave w computes slice Z0 + w:
-// its 16 blocks are 8 x-blocks by 2 y-blocks, lane l reads block n = l % 16
-// (x-block n % 8, y-block n / 8) and holds, after the MFMA, outputs
-// x0..x0+3 of row by = l / 16.  The lanes of one row then cover 32
-// consecutive x, so a wave's R load touches 8 rows: a layout whose wave
-// spans 4 columns and 64 rows took 2.3 times as long, bound by the cache
-// lines each load instruction touches.
+// A workgroup of 8 wavefronts owns a 32x16x8 output tile and stages D over
+// the tile and its patch halo, 36x20x12 (2.1 staged values per output), in
+// LDS (double buffered, one barrier per offset).  Wave w computes slice
+// Z0 + w in two halves of 8 rows, h = 0, 1, with one MFMA chain each (the
+// two chains interleaved): its 16 blocks per half are 8 x-blocks by 2
+// y-blocks, lane l reads block n = l % 16 (x-block n % 8, y-block n / 8)
+// and holds, after the MFMA, outputs x0..x0+3 of row 8 h + by, by = l / 16.
+// The lanes of one row then cover 32 consecutive x, so a wave's R load
+// touches 8 rows: a layout whose wave spans 4 columns and 64 rows took 2.3
+// times as long, bound by the cache lines each load instruction touches.
+// The 32x8x8 tile (one half per wave, 36x12x12 staged, 2.5 per output) took
+// 9-12% longer: the staging loads are the kernel's first limit.
 //
 // LDS layout: a region row is 8 windows of 8 columns, window k holding
 // region columns 4k .. 4k+7, so every block's window (region column 4 xb)
-// is window xb, 16-byte aligned.  Rows are 128 bytes apart and slices 1552
-// bytes (97 16-byte units, 1 mod 16).  Under gfx950's service groups for
+// is window xb, 16-byte aligned.  Rows are 128 bytes apart and slices 2576
+// bytes (161 16-byte units, 1 mod 16).  Under gfx950's service groups for
 // 16-byte accesses (16 lanes per cycle: {0-3,12-15,20-27}, {4-11,16-19,
 // 28-31} and the same + 32) every read of the plan and every staging write
 // hits 16 distinct 16-byte bank groups: a layout that wrote the shifted
 // half-windows with 8-byte stores spent half its LDS cycles in bank
 // conflicts, 7% slower than the row form.
 //
-// Staging: 9 slots of 64 lanes each cover a 4x4 block of region rows
-// (ry in 4 (s % 3) .. +3, rz in 4 (s / 3) .. +3); lane 4 j + c computes the D
+// Staging: 15 slots of 64 lanes each cover a 4x4 block of region rows
+// (ry in 4 (s % 5) .. +3, rz in 4 (s / 5) .. +3); lane 4 j + c computes the D
 // values of columns 8c .. 8c+7 of row (ry0 + (j>>1 & 1) + 2 (j>>3),
 // rz0 + (j>>2 & 1) + 2 parity(j & 7)), and lanes with c = 3 also of columns
 // 32-35.  It writes window 2c (its own 8 columns) and window 2c+1 (its upper
 // four and the next lane's lower four, moved by DPP; columns 32-35 for
-// c = 3).  Wave w runs slot w, and wave (workgroup index mod 8) slot 8.
+// c = 3).  Wave w runs slot w and, unless w = (workgroup index + 7) mod 8,
+// slot 8 + (w - workgroup index) mod 8.
 // ---------------------------------------------------------------------------
 // operand reads in flight ahead of the MFMA that consumes them
 #ifndef INLM_B4_KD
 #define INLM_B4_KD 2
 #endif
 namespace b4 {
-constexpr int TX = 32, TY = 8, TZ = 8;
+constexpr int TX = 32, TY = 16, TZ = 8;
+constexpr int NH = TY / 8;  // 32x8 output halves per wavefront
 constexpr int RX = TX + 4, RY = TY + 4, RZ = TZ + 4;  // staged region
 constexpr int ROW = 64;                               // LDS row pitch, halves (8 windows)
 constexpr int SLICE = RY * ROW + 8;                   // LDS slice pitch, halves (97 16-byte units)
 constexpr int LDS = RZ * SLICE;                       // halves per buffer
-static_assert(RX == 36 && RY % 4 == 0 && RZ % 4 == 0 && (RY / 4) * (RZ / 4) == 9, "staging slots");
+constexpr int NSLOT = (RY / 4) * (RZ / 4);
+static_assert(RX == 36 && RY % 4 == 0 && RZ % 4 == 0 && NSLOT > 8 && NSLOT <= 16, "staging slots");
 constexpr int NOPS = 7, NCLS = 5;
 // plane and row of the K group q of MFMA o; o = 4 holds rows 3, 4 of
 // plane +1 (q < 2) and of plane -1 (q >= 2)
@@ -1345,7 +1351,7 @@ __device__ __forceinline__ float b4_weight(const MfmaArgs<TS> &a, int dx, int dy
     return w;
 }
 
-// Its loop holds about 90 registers: two workgroups per CU (4 waves per SIMD)
+// Its loop holds up to 128 registers: two workgroups per CU (4 waves per SIMD)
 template <class TS>
 __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) void filter_mfma_b4(MfmaArgs<TS> a)
 {
@@ -1410,19 +1416,20 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     stg8.f = half2v{fh, fh};
     stg4.f = half2v{fh, fh};
     const int cg = lane & 3;
-    // one wave also runs slot 8; which one rotates with the workgroup, so
-    // that the workgroups sharing a CU put that work on different SIMDs
-    const bool two = wave == (int)(blockIdx.x & 7);
+    // slots 8-14 go to seven of the waves; the one left out rotates with the
+    // workgroup, so that the workgroups sharing a CU idle different SIMDs
+    const int jx = (wave + 8 - (int)(blockIdx.x & 7)) & 7;
+    const bool two = jx < b4::NSLOT - 8;
     auto slot_place = [&](int sl, int &coff, int &lrow) __attribute__((always_inline)) {
         const int j = lane >> 2;
-        const int ry = 4 * (sl % 3) + ((j >> 1) & 1) + 2 * (j >> 3);
-        const int rz = 4 * (sl / 3) + ((j >> 2) & 1) + 2 * (__builtin_popcount(j & 7) & 1);
+        const int ry = 4 * (sl % (b4::RY / 4)) + ((j >> 1) & 1) + 2 * (j >> 3);
+        const int rz = 4 * (sl / (b4::RY / 4)) + ((j >> 2) & 1) + 2 * (__builtin_popcount(j & 7) & 1);
         coff = (int)(rz * sp + (long)ry * g.nxp + 8 * cg);
         lrow = rz * b4::SLICE + ry * b4::ROW + 16 * cg;
     };
     int coffA, lrowA, coffX = 0, lrowX = 0;
     slot_place(wave, coffA, lrowA);
-    if (two) slot_place(8, coffX, lrowX);
+    if (two) slot_place(8 + jx, coffX, lrowX);
 
     // this lane's block and outputs
     const int n = lane & 15, xb = n & 7, yb = n >> 3;
@@ -1435,13 +1442,20 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int zc = z < g.nz ? z : g.nz - 1;
     const float sig = a.sigma[zc];
     const float mm = (-1.f / (sig * sig * a.beta)) * a.mscale;
+    const unsigned rhalf = (unsigned)(8 * g.nxp * (int)sizeof(TS));
     // D operand: block n's window xb (region columns 4 xb .. 4 xb + 7) of K
     // group q's row; MFMA o adds kOff[o], o = 4 its own per-group offset
     const int bbase = (wave + 2) * b4::SLICE + (4 * yb + 2 + q) * b4::ROW + 8 * xb;
     const int b4base = bbase + (q < 2 ? b4::SLICE + 3 * b4::ROW : -b4::SLICE + b4::ROW);
 
-    float fv[4] = {0.f, 0.f, 0.f, 0.f}, fw[4] = {0.f, 0.f, 0.f, 0.f};
-    ushort2w wmax2[2] = {{0, 0}, {0, 0}};
+    float fv[b4::NH][4], fw[b4::NH][4];
+    ushort2w wmax2[b4::NH][2];
+#pragma unroll
+    for (int h = 0; h < b4::NH; h++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) fv[h][i] = fw[h][i] = 0.f;
+        wmax2[h][0] = wmax2[h][1] = ushort2w{0, 0};
+    }
     const float dinv = 1.f / (pf * pf);
     const _Float16 msu1 = (_Float16)(mm * (1.f / 3.56648f) * dinv);
     const half2w msu = {msu1, msu1};
@@ -1531,25 +1545,36 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         if (two) store_slot(buf, pnX, p0X, pqX, q0X, lrowX);
     };
     struct RVals {
-        TS v[4];
+        TS v[b4::NH][4];
     };
     auto load_r = [&](RVals &rv, int lin) __attribute__((always_inline)) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc_r(lin), rbyte, soff_of(lin), 0);
-        __builtin_memcpy(rv.v, &v, 8);
-    };
-    auto accumulate = [&](const floatx4 &d, const RVals &rv) __attribute__((always_inline)) {
-        half2w w[2];
-        packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
-                       __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
+        const rsrc_t r = rsrc_r(lin);
+        const unsigned so = soff_of(lin);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float wi = (float)w[i >> 1][i & 1];
-            fv[i] = __builtin_fmaf(wi, (float)rv.v[i], fv[i]);
-            // times a runtime 1.0, so the sum stays one mixed-precision FMA
-            fw[i] = __builtin_fmaf(wi, one, fw[i]);
+        for (int h = 0; h < b4::NH; h++) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, rbyte + h * rhalf, so, 0);
+            __builtin_memcpy(rv.v[h], &v, 8);
         }
+    };
+    typedef floatx4 DVals[b4::NH];
+    auto accumulate = [&](const DVals &dd, const RVals &rv) __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 2; j++) wmax2[j] = __builtin_elementwise_max(wmax2[j], __builtin_bit_cast(ushort2w, w[j]));
+        for (int h = 0; h < b4::NH; h++) {
+            const floatx4 &d = dd[h];
+            half2w w[2];
+            packed_weights(__builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[0], d[1])),
+                           __builtin_bit_cast(half2w, __builtin_amdgcn_cvt_pkrtz(d[2], d[3])), msu, w[0], w[1]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float wi = (float)w[i >> 1][i & 1];
+                fv[h][i] = __builtin_fmaf(wi, (float)rv.v[h][i], fv[h][i]);
+                // times a runtime 1.0, so the sum stays one mixed-precision FMA
+                fw[h][i] = __builtin_fmaf(wi, one, fw[h][i]);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                wmax2[h][j] = __builtin_elementwise_max(wmax2[h][j], __builtin_bit_cast(ushort2w, w[j]));
+        }
     };
 
     const int nS = a.n_search;
@@ -1565,39 +1590,48 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     load_chunks(lin_at(1));
     __syncthreads();
 
-    floatx4 dA = {0.f, 0.f, 0.f, 0.f};
-    floatx4 dB = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+    DVals dA, dB;
     RVals rA, rB;
 #pragma unroll
-    for (int i = 0; i < 4; i++) rA.v[i] = rB.v[i] = TS(0.f);
+    for (int h = 0; h < b4::NH; h++) {
+        dA[h] = floatx4{0.f, 0.f, 0.f, 0.f};
+        dB[h] = floatx4{__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+#pragma unroll
+        for (int i = 0; i < 4; i++) rA.v[h][i] = rB.v[h][i] = TS(0.f);
+    }
     constexpr int kBlock = 62;
     int linv = 0, kb = 0;
-    auto step = [&](int k, const _Float16 *rd, _Float16 *st, floatx4 &dout, RVals &rout, const floatx4 &din,
+    auto step = [&](int k, const _Float16 *rd, _Float16 *st, DVals &dout, RVals &rout, const DVals &din,
                     const RVals &rin) __attribute__((always_inline)) {
         const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
         const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
         load_r(rout, l0);
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-        constexpr int NR = b4::NOPS;
+        DVals acc;
+#pragma unroll
+        for (int h = 0; h < b4::NH; h++) acc[h] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // the halves' chains interleaved: operation r is MFMA r / NH of half r % NH
+        constexpr int NR = b4::NOPS * b4::NH;
         constexpr int KD = INLM_B4_KD;
-        auto b_of = [&](auto oi) __attribute__((always_inline)) {
-            constexpr int o = decltype(oi)::value;
+        auto b_of = [&](auto ri) __attribute__((always_inline)) {
+            constexpr int o = decltype(ri)::value / b4::NH, h = decltype(ri)::value % b4::NH;
             if constexpr (o == 4)
-                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + b4base, 16));
+                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + b4base + h * 8 * b4::ROW, 16));
             else
-                return *reinterpret_cast<const half8 *>(__builtin_assume_aligned(rd + bbase + b4::kOff[o], 16));
+                return *reinterpret_cast<const half8 *>(
+                    __builtin_assume_aligned(rd + bbase + h * 8 * b4::ROW + b4::kOff[o], 16));
         };
         half8 ab[KD];
         mstatic_for<0, KD>([&](auto ri) __attribute__((always_inline)) { ab[decltype(ri)::value] = b_of(ri); });
         mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
             constexpr int r = decltype(ri)::value;
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[b4::kCls[r]], ab[r % KD], acc, 0, 0, 0);
+            constexpr int o = r / b4::NH, h = r % b4::NH;
+            acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[b4::kCls[o]], ab[r % KD], acc[h], 0, 0, 0);
             if constexpr (r + KD < NR) ab[r % KD] = b_of(std::integral_constant<int, r + KD>{});
         });
         accumulate(din, rin);
         stage(st);
-        constexpr int NV = (2 * kWeightValuPacked + kStageValu<TS, _Float16> * 3 + NR - 2) / (NR - 1);
-        __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
+        constexpr int NV = (2 * b4::NH * kWeightValuPacked + kStageValu<TS, _Float16> * 3 * 2 + NR - 2) / (NR - 1);
+        __builtin_amdgcn_sched_group_barrier(0x0020, b4::NH, 0);
         __builtin_amdgcn_sched_group_barrier(0x0100, KD, 0);
         mstatic_for<0, NR>([&](auto ri) __attribute__((always_inline)) {
             __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);
@@ -1607,7 +1641,8 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __builtin_amdgcn_sched_barrier(0);
         load_chunks(l2);
         __syncthreads();
-        dout = acc;
+#pragma unroll
+        for (int h = 0; h < b4::NH; h++) dout[h] = acc[h];
     };
     for (kb = 0; kb < nS; kb += kBlock) {
         linv = lin_at(kb + lane);
@@ -1621,22 +1656,27 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
     if ((nS & 1) == 0) accumulate(dB, rB);
 
-    float wmax[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) wmax[i] = (float)__builtin_bit_cast(half2w, wmax2[i >> 1])[i & 1];
-    if (y < g.ny && z < g.nz) {
+    for (int h = 0; h < b4::NH; h++) {
+        const int yh = y + 8 * h;
+        float wmax[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int x = x0 + i;
-            if (x < g.nx) {
-                const long o = ((long)z * g.ny + y) * g.nx + x;
-                float res = a.raw[o];
-                if (wmax[i] > 0.f) {
-                    const double r0 = (double)(float)Rv[i];
-                    const double v = ((double)fv[i] + (double)wmax[i] * r0) / ((double)fw[i] + (double)wmax[i]);
-                    res = (float)(a.shift + v * a.inv_scale);
+        for (int i = 0; i < 4; i++) wmax[i] = (float)__builtin_bit_cast(half2w, wmax2[h][i >> 1])[i & 1];
+        if (yh < g.ny && z < g.nz) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + i;
+                if (x < g.nx) {
+                    const long o = ((long)z * g.ny + yh) * g.nx + x;
+                    float res = a.raw[o];
+                    if (wmax[i] > 0.f) {
+                        const double r0 = (double)(float)Rv[(long)8 * h * g.nxp + i];
+                        const double v =
+                            ((double)fv[h][i] + (double)wmax[i] * r0) / ((double)fw[h][i] + (double)wmax[i]);
+                        res = (float)(a.shift + v * a.inv_scale);
+                    }
+                    a.out[o] = res;
                 }
-                a.out[o] = res;
             }
         }
     }

@@ -271,7 +271,7 @@ def test_fast_forms_zvarying_sigma(gpu_available, form, rp):
                                       ((67, 18, 9), (7, 7, 5))])
 def test_block_kernel_equals_row_kernel(gpu_available, form, shape, rs):
     """At patch radius 2 the 16-bit forms run the 4x4-block kernel (seven
-    MFMAs per offset over 16x16x8 tiles); the row kernel (kernel_select 2,
+    MFMAs per half over 32x16x8 tiles); the row kernel (kernel_select 2,
     eleven MFMAs over 32x8x8 tiles) stages the same FP16 planes with the same
     per-tile scale, so the two differ only in the order of FP32 sums: ragged
     shapes, both storage types."""
