@@ -94,6 +94,7 @@ struct inlm_handle {
     inlm::Geometry geo{};
     inlm::FilterGeom fg{};
     std::vector<inlm::SearchEntry> search, search_fast;
+    std::vector<unsigned char> search_fast_blob;  // the fast order and its staging moves, as uploaded
     std::vector<int> tap_dxyz;
     inlm::TapTable taps{};
     int rp_tiled = 0;
@@ -251,7 +252,7 @@ size_t handle_bytes(const inlm_params &pin, const int32_t shape_in[3])
     size_t b = 2 * padb + sizeof(float) * vox + sizeof(float) * (size_t)g.nz;
     if (fast) b += sizeof(float) * vox + inlm::kMomentDoubles * sizeof(double);
     const size_t ns = search_offsets(p.radius_searchspace, p.nslices).size();
-    b += 2 * ns * sizeof(SearchEntry);
+    b += 2 * ns * sizeof(SearchEntry) + (ns + kStageMovesPad) * sizeof(int);
     if (p.noise_mode != INLM_NOISE_MANUAL && p.n_samples > 0) b += (size_t)p.n_samples * (sizeof(int64_t) + sizeof(double));
     return b;
 }
@@ -447,11 +448,25 @@ inlm_status inlm_create(const inlm_params *pin, const int32_t shape_in[3], int32
         if (a.dy != b.dy) return a.dy < b.dy;
         return a.dx < b.dx;
     });
-    if ((e = hipMalloc(&h->search_fast_dev, sizeof(inlm::SearchEntry) * h->search_fast.size())) != hipSuccess)
-        return bail(hip_fail(nullptr, e, "hipMalloc(search)"));
-    if ((e = hipMemcpyAsync(h->search_fast_dev, h->search_fast.data(), sizeof(inlm::SearchEntry) * h->search_fast.size(),
-                            hipMemcpyHostToDevice, h->stream)) != hipSuccess)
-        return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
+    // followed by the block kernel's staging moves (kStageMovesPad)
+    {
+        const size_t ns = h->search_fast.size();
+        std::vector<int> moves(ns + inlm::kStageMovesPad, 0);
+        for (size_t k = 1; k < ns; k++) {
+            const inlm::SearchEntry &c = h->search_fast[k], &b = h->search_fast[k - 1];
+            if (c.dz == b.dz && c.dy == b.dy && c.dx == b.dx + 2 && moves[k - 1] < inlm::kStageMovesMax)
+                moves[k] = moves[k - 1] + 1;
+        }
+        const size_t tb = sizeof(inlm::SearchEntry) * ns, mb = sizeof(int) * moves.size();
+        h->search_fast_blob.resize(tb + mb);
+        std::memcpy(h->search_fast_blob.data(), h->search_fast.data(), tb);
+        std::memcpy(h->search_fast_blob.data() + tb, moves.data(), mb);
+        if ((e = hipMalloc(&h->search_fast_dev, tb + mb)) != hipSuccess)
+            return bail(hip_fail(nullptr, e, "hipMalloc(search)"));
+        if ((e = hipMemcpyAsync(h->search_fast_dev, h->search_fast_blob.data(), tb + mb, hipMemcpyHostToDevice,
+                                h->stream)) != hipSuccess)
+            return bail(hip_fail(nullptr, e, "hipMemcpy(search)"));
+    }
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(hip_fail(nullptr, e, "sync"));
     (void)st;
     *out = h;

@@ -101,6 +101,15 @@ hipError_t launch_pad_normalise(int form, const float *in, void *out, long shift
 // clearing, and the result does not depend on scheduling.
 constexpr int kMomentBlocks = 1024;
 constexpr int kMomentDoubles = 2 + 2 * kMomentBlocks;
+
+// The full-search order of the matrix-core forms is followed on the device by
+// n_search + kStageMovesPad ints, one per offset and zero past the last: 0
+// where the block kernel loads the staging chunks of a wavefront's own slot,
+// s = 1 .. kStageMovesMax where the offset is the one before it moved by
+// dx + 2 within one dz and dy (one dword per region row), s such moves after
+// the last load.
+constexpr int kStageMovesPad = 64;
+constexpr int kStageMovesMax = 2;
 hipError_t launch_moments(const float *in, long n, double *acc, hipStream_t st);
 inline size_t form_elem_size(int form) { return form == FORM_FP32 || form == FORM_FP32_FAST ? 4 : 2; }
 

@@ -1306,7 +1306,14 @@ void filter_mfma(MfmaArgs<TS> a)
 // 32-35.  It writes window 2c (its own 8 columns) and window 2c+1 (its upper
 // four and the next lane's lower four, moved by DPP; columns 32-35 for
 // c = 3).  Wave w runs slot w and, unless w = (workgroup index + 7) mod 8,
-// slot 8 + (w - workgroup index) mod 8.
+// slot 8 + (w - workgroup index) mod 8.  In the FP16 form slot w's chunks
+// are loaded only at every third offset of a run of the full search (one dz
+// and dy, dx stepping by 2) and moved one dword along the row in between
+// (the cg = 3 lanes load columns 32-39 for that), which skips about half of
+// slot w's loads at search radius 7: 3-4% faster.  The second slot keeps
+// loading at every offset (its moves would need four more registers than
+// the loop has), and so does the BF16 form, whose staging arithmetic no
+// longer interleaved with the MFMA chain once the move was added (7% slower).
 // ---------------------------------------------------------------------------
 // operand reads in flight ahead of the MFMA that consumes them
 #ifndef INLM_B4_KD
@@ -1486,16 +1493,26 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     auto soff_of = [&](int lin) __attribute__((always_inline)) {
         return (unsigned)((lin - min_lin) * (int)sizeof(TS));
     };
-    // per slot: 8 columns, and columns 32-35 on lanes with cg = 3 (at 16
-    // bytes past their 8-column chunk)
+    // per slot: 8 columns, and the row's tail on lanes with cg = 3 (at 16
+    // bytes past their 8-column chunk): columns 32-35, or 32-39 for slot A of
+    // the FP16 form, which moves its chunks along runs of the search instead
+    // of loading them at every offset (below)
+    constexpr bool kMove = std::is_same<TS, _Float16>::value;
     chunk8 pnA, pnX;
-    chunk4 pqA = {}, pqX = {};
-    auto load_chunks = [&](int lin) __attribute__((always_inline)) {
+    std::conditional_t<kMove, chunk8, chunk4> pqA = {};
+    chunk4 pqX = {};
+    auto load_chunks = [&](int lin, bool slot_a) __attribute__((always_inline)) {
         const rsrc_t r = rsrc_p(lin);
         const unsigned so = soff_of(lin);
         const unsigned oA = (unsigned)(coffA * (int)sizeof(TS)), oX = (unsigned)(coffX * (int)sizeof(TS));
-        stg8.load(pnA, r, oA, so);
-        if (cg == 3) stg4.load(pqA, r, oA + 8 * (unsigned)sizeof(TS), so);
+        if (slot_a) {
+            stg8.load(pnA, r, oA, so);
+            if constexpr (kMove) {
+                if (cg == 3) stg8.load(pqA, r, oA + 8 * (unsigned)sizeof(TS), so);
+            } else {
+                if (cg == 3) stg4.load(pqA, r, oA + 8 * (unsigned)sizeof(TS), so);
+            }
+        }
         if (two) {
             stg8.load(pnX, r, oX, so);
             if (cg == 3) stg4.load(pqX, r, oX + 8 * (unsigned)sizeof(TS), so);
@@ -1505,7 +1522,7 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // difference, search offset 0), scaled by -f, in registers
     chunk8 p0A, p0X;
     chunk4 q0A, q0X;
-    load_chunks(0);
+    load_chunks(0, true);
     __builtin_memcpy(&p0A, &pnA, sizeof(p0A));
     __builtin_memcpy(&q0A, &pqA, sizeof(q0A));
     stg8.own_scaled(p0A);
@@ -1541,8 +1558,32 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         st16(buf + lrow + 8, e);
     };
     auto stage = [&](_Float16 *buf) __attribute__((always_inline)) {
-        store_slot(buf, pnA, p0A, pqA, q0A, lrowA);
+        chunk4 pq;
+        __builtin_memcpy(&pq, &pqA, sizeof(pq));
+        store_slot(buf, pnA, p0A, pq, q0A, lrowA);
         if (two) store_slot(buf, pnX, p0X, pqX, q0X, lrowX);
+    };
+    // Slot A's chunks of an offset from those of the offset before it when
+    // the two differ by dx + 2 (same dz and dy): each region row moves by one
+    // dword, so dwords 1-3 move down and the last comes from the next lane of
+    // the row (DPP row shift) or, for cg = 3, from the tail, which holds
+    // columns 32-39 of the loaded offset and so serves two such moves.
+    auto move_a = [&]() __attribute__((always_inline)) {
+        if constexpr (kMove) {
+            unsigned w[4], tl[4];
+            __builtin_memcpy(w, &pnA, 16);
+            __builtin_memcpy(tl, &pqA, 16);
+            const unsigned from_next = (unsigned)__builtin_amdgcn_mov_dpp((int)w[0], 0x101, 0xf, 0xf, true);
+            w[0] = w[1];
+            w[1] = w[2];
+            w[2] = w[3];
+            w[3] = cg == 3 ? tl[0] : from_next;
+            tl[0] = tl[1];
+            tl[1] = tl[2];
+            tl[2] = tl[3];
+            __builtin_memcpy(&pnA, w, 16);
+            __builtin_memcpy(&pqA, tl, 16);
+        }
     };
     struct RVals {
         TS v[b4::NH][4];
@@ -1579,15 +1620,19 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 
     const int nS = a.n_search;
     auto lin_at = [&](int k) __attribute__((always_inline)) { return a.search_full[k < nS ? k : nS - 1].lin; };
+    // per offset of the full search, after the table: 0 to load slot A's
+    // chunks, s > 0 to move those of the offset before it (kStageMovesMax)
+    const int *moves = reinterpret_cast<const int *>(a.search_full + nS);
+    auto loads_a = [&](int mv) __attribute__((always_inline)) { return !kMove || mv == 0; };
 
     // Software pipeline as in filter_mfma: step k runs the MFMA chain of
     // offset k right after the barrier, with the weights of offset k-1 and
     // the staging arithmetic of offset k+1 interleaved, writes that plane to
     // the idle buffer and loads the chunks of offset k+2.
     __syncthreads();
-    load_chunks(lin_at(0));
+    load_chunks(lin_at(0), true);
     stage(lds[0]);
-    load_chunks(lin_at(1));
+    load_chunks(lin_at(1), loads_a(nS > 1 ? moves[1] : 0));
     __syncthreads();
 
     DVals dA, dB;
@@ -1600,11 +1645,13 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         for (int i = 0; i < 4; i++) rA.v[h][i] = rB.v[h][i] = TS(0.f);
     }
     constexpr int kBlock = 62;
-    int linv = 0, kb = 0;
+    int linv = 0, mvv = 0, kb = 0;
     auto step = [&](int k, const _Float16 *rd, _Float16 *st, DVals &dout, RVals &rout, const DVals &din,
                     const RVals &rin) __attribute__((always_inline)) {
         const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
         const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
+        const int m1 = kMove ? __builtin_amdgcn_readlane(mvv, k + 1 - kb) : 0;
+        const int m2 = kMove ? __builtin_amdgcn_readlane(mvv, k + 2 - kb) : 0;
         load_r(rout, l0);
         DVals acc;
 #pragma unroll
@@ -1629,6 +1676,7 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             if constexpr (r + KD < NR) ab[r % KD] = b_of(std::integral_constant<int, r + KD>{});
         });
         accumulate(din, rin);
+        if (!loads_a(m1)) move_a();
         stage(st);
         constexpr int NV = (2 * b4::NH * kWeightValuPacked + kStageValu<TS, _Float16> * 3 * 2 + NR - 2) / (NR - 1);
         __builtin_amdgcn_sched_group_barrier(0x0020, b4::NH, 0);
@@ -1639,13 +1687,14 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             if constexpr (decltype(ri)::value >= 1) __builtin_amdgcn_sched_group_barrier(0x0002, NV, 0);
         });
         __builtin_amdgcn_sched_barrier(0);
-        load_chunks(l2);
+        load_chunks(l2, loads_a(m2));
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < b4::NH; h++) dout[h] = acc[h];
     };
     for (kb = 0; kb < nS; kb += kBlock) {
         linv = lin_at(kb + lane);
+        if constexpr (kMove) mvv = kb + lane < nS ? moves[kb + lane] : 0;
         const int ke = kb + kBlock < nS ? kb + kBlock : nS;
         for (int k = kb; k < ke; k += 2) {
             step(k, lds[0], lds[1], dA, rA, dB, rB);
