@@ -132,12 +132,16 @@ __global__ __launch_bounds__(256) void conv16_mfma(const __bf16 *__restrict__ X,
 // Weight gradients on the vector ALUs (FP32): dW[tap][k][n] += sum_v
 // X[v + tap][k] * G[v][n] and dB[n] += sum_v G[v][n].  A block of 256 threads
 // = (k, n) pairs takes a chunk of 64 voxels of one row at a time through LDS.
-// Cin is 16 (BF16 activations) or 1 (the FP32 input, layer 1).
+// Cin is 16 (BF16 activations) or 1 (the FP32 input, layer 1).  Each of the
+// kWgradBlocks blocks writes its partial sums (part[block][tap][k][n], then
+// part[block][NTAP CIN C + n] for the bias) and wgrad_reduce adds them in
+// block order, so training repeats bit for bit.
 // ---------------------------------------------------------------------------
+constexpr int kWgradBlocks = 2048;
+constexpr int kWgradStride = NTAP * C * C + C;  // floats per block's partials
 template <int CIN>
 __global__ __launch_bounds__(256) void conv_wgrad(const void *__restrict__ Xv, const __bf16 *__restrict__ G,
-                                                  float *__restrict__ dW, float *__restrict__ dB, ActGeom g,
-                                                  int rows_per_block)
+                                                  float *__restrict__ part, ActGeom g, int rows_per_block)
 {
     constexpr int CH = 64;                   // voxels per chunk along x
     __shared__ float xs[NTAP][CH][CIN];     // X at the 27 shifted positions
@@ -180,10 +184,24 @@ __global__ __launch_bounds__(256) void conv_wgrad(const void *__restrict__ Xv, c
             }
         }
     }
+    float *pb = part + (long)blockIdx.x * kWgradStride;
     if (CIN == C || tid < C) {
-        for (int t = 0; t < NTAP; t++) atomicAdd(&dW[((long)t * CIN + (CIN == 1 ? 0 : k)) * C + n], acc[t]);
-        if (CIN == 1 || k == 0) atomicAdd(&dB[n], accb);
+        for (int t = 0; t < NTAP; t++) pb[(t * CIN + (CIN == 1 ? 0 : k)) * C + n] = acc[t];
+        if (CIN == 1 || k == 0) pb[NTAP * CIN * C + n] = accb;
     }
+}
+
+// dW[e] += sum over blocks b (in order) of part[b][e], e < NTAP CIN C, and
+// dB likewise from the bias partials
+__global__ __launch_bounds__(256) void wgrad_reduce(const float *__restrict__ part, int nblocks, int nw,
+                                                    float *__restrict__ dW, float *__restrict__ dB)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nw + C) return;
+    float s = 0.f;
+    for (int b = 0; b < nblocks; b++) s += part[(long)b * kWgradStride + e];
+    if (e < nw) dW[e] += s;
+    else dB[e - nw] += s;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +279,7 @@ __device__ __forceinline__ float block_reduce(float v, float *sh, bool is_max)
 __global__ __launch_bounds__(256) void head_fwd(const float *__restrict__ feat, const float *__restrict__ Wh,
                                                 const float *__restrict__ bh, int ns, const float *__restrict__ mass,
                                                 float *__restrict__ probs, float *__restrict__ dlogits,
-                                                float *__restrict__ loss, float inv_tiles)
+                                                float *__restrict__ tile_loss, float inv_tiles)
 {
     extern __shared__ float lg[];  // ns logits
     __shared__ float sh[4];
@@ -299,11 +317,23 @@ __global__ __launch_bounds__(256) void head_fwd(const float *__restrict__ feat, 
     }
     if (mass) {
         ce = block_reduce(ce, sh, false);
-        if (threadIdx.x == 0) atomicAdd(loss, ce * inv_tiles);
+        if (threadIdx.x == 0) tile_loss[t] = ce * inv_tiles;
     }
 }
 
 // dWh[c][s] = sum_t feat[t][c] dl[t][s]; dbh[s] = sum_t dl[t][s]
+// loss = sum of the tiles' terms, in a fixed order (strided per thread,
+// then the block's fixed reduction tree)
+__global__ __launch_bounds__(256) void loss_reduce(const float *__restrict__ tile_loss, int ntiles,
+                                                   float *__restrict__ loss)
+{
+    __shared__ float sh[4];
+    float v = 0.f;
+    for (int t = threadIdx.x; t < ntiles; t += 256) v += tile_loss[t];
+    v = block_reduce(v, sh, false);
+    if (threadIdx.x == 0) loss[0] = v;
+}
+
 __global__ __launch_bounds__(256) void head_bwd_w(const float *__restrict__ feat, const float *__restrict__ dl,
                                                   int ntiles, int ns, float *__restrict__ dWh, float *__restrict__ dbh)
 {
@@ -473,7 +503,9 @@ struct ProposalNet::Impl {
     __bf16 *gbuf[2] = {nullptr, nullptr};
     long cap_tiles = 0;
     float *feat = nullptr, *dfeat = nullptr, *dl = nullptr, *probs = nullptr, *loss = nullptr;
+    float *tile_loss = nullptr;  // per tile, summed by loss_reduce
     double *moments = nullptr;
+    float *wpart = nullptr;  // weight-gradient partials, kWgradBlocks x kWgradStride
     int steps = 0;
 };
 
@@ -527,12 +559,14 @@ hipError_t ensure_acts(ProposalNet::Impl &m, const int shape[3], bool training)
         (void)hipFree(m.dfeat);
         (void)hipFree(m.dl);
         (void)hipFree(m.probs);
-        m.feat = m.dfeat = m.dl = m.probs = nullptr;
+        (void)hipFree(m.tile_loss);
+        m.feat = m.dfeat = m.dl = m.probs = m.tile_loss = nullptr;
         m.cap_tiles = 0;
         if ((e = hipMalloc(&m.feat, tiles * C * sizeof(float))) != hipSuccess) return e;
         if ((e = hipMalloc(&m.dfeat, tiles * C * sizeof(float))) != hipSuccess) return e;
         if ((e = hipMalloc(&m.dl, tiles * (long)m.ns * sizeof(float))) != hipSuccess) return e;
         if ((e = hipMalloc(&m.probs, tiles * (long)m.ns * sizeof(float))) != hipSuccess) return e;
+        if ((e = hipMalloc(&m.tile_loss, tiles * sizeof(float))) != hipSuccess) return e;
         m.cap_tiles = tiles;
     }
     return hipSuccess;
@@ -546,7 +580,8 @@ ProposalNet::~ProposalNet()
 {
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    for (float *p : {d->params, d->grads, d->m1, d->m2, d->in, d->feat, d->dfeat, d->dl, d->probs, d->loss})
+    for (float *p : {d->params, d->grads, d->m1, d->m2, d->in, d->feat, d->dfeat, d->dl, d->probs, d->loss,
+                     d->tile_loss})
         (void)hipFree(p);
     for (int l = 0; l < 3; l++) {
         (void)hipFree(d->wf[l]);
@@ -555,6 +590,7 @@ ProposalNet::~ProposalNet()
     for (auto &a : d->act) (void)hipFree(a);
     for (auto &a : d->gbuf) (void)hipFree(a);
     (void)hipFree(d->moments);
+    (void)hipFree(d->wpart);
     if (d->st) (void)hipStreamDestroy(d->st);
     delete d;
 }
@@ -594,6 +630,7 @@ hipError_t ProposalNet::init(int ns, int device, uint64_t seed)
     if ((e = hipMalloc(&d->m2, bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d->loss, sizeof(float))) != hipSuccess) return e;
     if ((e = hipMalloc(&d->moments, kMomentDoubles * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&d->wpart, (size_t)kWgradBlocks * kWgradStride * sizeof(float))) != hipSuccess) return e;
     for (int l = 0; l < 3; l++) {
         if ((e = hipMalloc(&d->wf[l], NTAP * C * C * sizeof(__bf16))) != hipSuccess) return e;
         if ((e = hipMalloc(&d->wb[l], NTAP * C * C * sizeof(__bf16))) != hipSuccess) return e;
@@ -691,9 +728,10 @@ hipError_t ProposalNet::run(const float *vol, const int shape[3], const float *m
         hipLaunchKernelGGL(conv16_mfma, dim3(cb), dim3(256), 0, m.st, A(l), m.wf[l], P + m.off_b[l],
                            (const __bf16 *)nullptr, A(l + 1), g, 0);
     hipLaunchKernelGGL(pool_fwd, dim3(ntiles), dim3(256), 0, m.st, A(3), m.feat, g, ntx, nty);
-    if ((e = hipMemsetAsync(m.loss, 0, sizeof(float), m.st)) != hipSuccess) return e;
     hipLaunchKernelGGL(head_fwd, dim3(ntiles), dim3(256), m.ns * sizeof(float), m.st, m.feat, P + m.off_wh,
-                       P + m.off_bh, m.ns, mass, m.probs, mass ? m.dl : nullptr, m.loss, 1.f / (float)ntiles);
+                       P + m.off_bh, m.ns, mass, m.probs, mass ? m.dl : nullptr, m.tile_loss, 1.f / (float)ntiles);
+    if (mass) hipLaunchKernelGGL(loss_reduce, dim3(1), dim3(256), 0, m.st, m.tile_loss, ntiles, m.loss);
+    else if ((e = hipMemsetAsync(m.loss, 0, sizeof(float), m.st)) != hipSuccess) return e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (training) {
         float *G = m.grads;
@@ -703,19 +741,28 @@ hipError_t ProposalNet::run(const float *vol, const int shape[3], const float *m
         hipLaunchKernelGGL(head_bwd_x, dim3(ntiles), dim3(256), 0, m.st, m.dl, P + m.off_wh, m.ns, m.dfeat);
         // gradient at the pre-activation of the last convolution
         hipLaunchKernelGGL(pool_bwd, dim3(vb), dim3(256), 0, m.st, m.dfeat, m.act[3], m.gbuf[0], g, ntx, nty);
-        const int rpb = 8;
-        const unsigned wgb = (unsigned)((g.ny * (long)g.nz + rpb - 1) / rpb);
+        // a fixed number of blocks over the rows, each writing its partials
+        const long nrows = g.ny * (long)g.nz;
+        const int rpb = (int)((nrows + kWgradBlocks - 1) / kWgradBlocks);
+        const unsigned wgb = (unsigned)((nrows + rpb - 1) / rpb);
+        auto reduce = [&](int cin, float *dW, float *dB) {
+            const int nw = NTAP * cin * C;
+            hipLaunchKernelGGL(wgrad_reduce, dim3((unsigned)((nw + C + 255) / 256)), dim3(256), 0, m.st, m.wpart,
+                               (int)wgb, nw, dW, dB);
+        };
         int cur = 0;
         for (int l = 2; l >= 0; l--) {
             // convolution l+2: input act[l], gradient at its output gbuf[cur]
             hipLaunchKernelGGL(conv_wgrad<C>, dim3(wgb), dim3(256), 0, m.st, (const void *)m.act[l], m.gbuf[cur],
-                               G + m.off_w[l], G + m.off_b[l], g, rpb);
+                               m.wpart, g, rpb);
+            reduce(C, G + m.off_w[l], G + m.off_b[l]);
             hipLaunchKernelGGL(conv16_mfma, dim3(cb), dim3(256), 0, m.st, m.gbuf[cur], m.wb[l],
                                (const float *)nullptr, m.act[l], m.gbuf[cur ^ 1], g, 1);
             cur ^= 1;
         }
-        hipLaunchKernelGGL(conv_wgrad<1>, dim3(wgb), dim3(256), 0, m.st, (const void *)m.in, m.gbuf[cur],
-                           G + m.off_w1, G + m.off_b1, g, rpb);
+        hipLaunchKernelGGL(conv_wgrad<1>, dim3(wgb), dim3(256), 0, m.st, (const void *)m.in, m.gbuf[cur], m.wpart,
+                           g, rpb);
+        reduce(1, G + m.off_w1, G + m.off_b1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (loss_host) {

@@ -1693,8 +1693,13 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         for (int h = 0; h < b4::NH; h++) dout[h] = acc[h];
     };
     for (kb = 0; kb < nS; kb += kBlock) {
-        linv = lin_at(kb + lane);
-        if constexpr (kMove) mvv = kb + lane < nS ? moves[kb + lane] : 0;
+        // the lane index again (mbcnt of a full mask), so that the table
+        // addresses are formed here rather than held in registers (or
+        // spilled) through the inner loop; volatile, so it is not hoisted
+        int ln;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(ln));
+        linv = lin_at(kb + ln);
+        if constexpr (kMove) mvv = kb + ln < nS ? moves[kb + ln] : 0;
         const int ke = kb + kBlock < nS ? kb + kBlock : nS;
         for (int k = kb; k < ke; k += 2) {
             step(k, lds[0], lds[1], dA, rA, dB, rB);

@@ -119,6 +119,25 @@ def test_adam_update(gpu_available):
     assert losses[-1] < losses[0]
 
 
+def test_training_repeats_bit_for_bit(gpu_available):
+    """The weight gradients are summed in a fixed order (per-block partials,
+    then one reduction), so two networks of one seed trained on the same
+    data end with identical parameters, gradients and losses."""
+    ns, shape, vol, mass = _net_case(3)
+    nz, ny, nx = shape
+    vd = torch.from_numpy(vol).cuda()
+    md = torch.from_numpy(mass).cuda()
+    runs = []
+    for rep in range(2):
+        print(f"[gpu] training run {rep}", flush=True)
+        net = PR.OffsetProposal(ns, seed=13)
+        losses = [net.train_step(vd.data_ptr(), [nx, ny, nz], md.data_ptr(), 3e-3) for _ in range(20)]
+        runs.append((np.array(losses, np.float32), net.grads(), net.get_params()))
+        net.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(bits(a), bits(b))
+
+
 # max, rms error against the FP64 oracle in noise-sigma units.  FP32_FAST
 # keeps the volumes in FP32 (FP16 difference planes only) and is held to the
 # full-search budget; with 40 of 114 offsets a voxel's output rests on fewer
