@@ -16,9 +16,12 @@
 // Volumes are stored normalised, v_s = (v - shift) * scale, which leaves
 // every weight unchanged once the per-slice multiplier is divided by scale^2.
 //
-// A workgroup (8 wavefronts) owns a 32x8x8 output tile and stages D_s over
-// the tile plus the patch halo in LDS (double buffered, one barrier per
-// offset); wavefront w computes the 16x16 block x-half (w & 1), z pair (w >> 1).
+// This row form (filter_mfma: every patch radius, the FP8-plane and FP32_FAST
+// forms, the pruned per-tile lists): a workgroup (8 wavefronts) owns a
+// 32x8x8 output tile and stages D_s over the tile plus the patch halo in LDS
+// (double buffered, one barrier per offset); wavefront w computes the 16x16
+// block x-half (w & 1), z pair (w >> 1).  The full search at patch radius 2
+// in the 16-bit forms runs the block form (filter_mfma_b4, further below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1303,20 +1306,21 @@ void filter_mfma(MfmaArgs<TS> a)
 // Staging: 15 slots of 64 lanes each cover a 4x4 block of region rows
 // (ry in 4 (s % 5) .. +3, rz in 4 (s / 5) .. +3); lane 4 j + c computes the D
 // values of columns 8c .. 8c+7 of row (ry0 + j / 4, rz0 + j % 4), and lanes
-// with c = 3 also of columns 32-35.  The two rows of 8 consecutive lanes
+// with c = 3 also of columns 32-35.  It writes window 2c (its own 8 columns)
+// and window 2c+1 (its upper four and the next lane's lower four, moved by
+// DPP; columns 32-35 for c = 3).  The two rows of 8 consecutive lanes
 // (j = 2i, 2i+1) are one slice apart, an odd number of 16-byte units, so
-// their four chunks interleave in the banks; an earlier mapping with the
-// two rows two slices apart wrote with 2-way conflicts (2% slower).  It writes window 2c (its own 8 columns) and window 2c+1 (its upper
-// four and the next lane's lower four, moved by DPP; columns 32-35 for
-// c = 3).  Wave w runs slot w and, unless w = (workgroup index + 7) mod 8,
-// slot 8 + (w - workgroup index) mod 8.  In the FP16 form slot w's chunks
-// are loaded only at every third offset of a run of the full search (one dz
-// and dy, dx stepping by 2) and moved one dword along the row in between
-// (the cg = 3 lanes load columns 32-39 for that), which skips about half of
-// slot w's loads at search radius 7: 3-4% faster.  The second slot keeps
-// loading at every offset (its moves would need four more registers than
-// the loop has), and so does the BF16 form, whose staging arithmetic no
-// longer interleaved with the MFMA chain once the move was added (7% slower).
+// their chunks interleave in the banks; an earlier mapping with the two rows
+// two slices apart wrote with 2-way conflicts (2% slower).  Wave w runs
+// slot w and, unless w = (workgroup index + 7) mod 8, slot
+// 8 + (w - workgroup index) mod 8.
+//
+// Run moves (FP16 form): slot w's chunks are loaded only at every third
+// offset of a run of the full search (one dz and dy, dx stepping by 2) and
+// moved one dword along the row in between (the cg = 3 lanes load columns
+// 32-39 for that), which skips about half of slot w's loads at search
+// radius 7: 3-4% faster.  Moving the second slot as well, or moving in the
+// BF16 form, puts a spill reload into the loop and was slower (3% and 7%).
 // ---------------------------------------------------------------------------
 // operand reads in flight ahead of the MFMA that consumes them
 #ifndef INLM_B4_KD
