@@ -268,13 +268,15 @@ def test_fast_forms_zvarying_sigma(gpu_available, form, rp):
 
 @pytest.mark.parametrize("form", ["fp16", "bf16"])
 @pytest.mark.parametrize("shape,rs", [((40, 30, 20), (3, 3, 3)), ((41, 35, 13), (4, 4, 2)), ((16, 16, 8), (2, 2, 2)),
-                                      ((67, 18, 9), (7, 7, 5))])
+                                      ((67, 18, 9), (7, 7, 5)), ((70, 33, 17), (7, 3, 2)), ((24, 20, 10), (1, 1, 1))])
 def test_block_kernel_equals_row_kernel(gpu_available, form, shape, rs):
     """At patch radius 2 the 16-bit forms run the 4x4-block kernel (seven
     MFMAs per half over 32x16x8 tiles); the row kernel (kernel_select 2,
     eleven MFMAs over 32x8x8 tiles) stages the same FP16 planes with the same
     per-tile scale, so the two differ only in the order of FP32 sums: ragged
-    shapes, both storage types."""
+    shapes, both storage types.  The search shapes give the FP16 form's run
+    moves runs of every length from 1 (radius 1: no moves) to 8 (radius 7),
+    and an anisotropic search."""
     spec = ph.PhantomSpec(shape, seed=53, radii=(2, 6), noise_sigma=1500.0, blur_sigma=0.8, streaks=2,
                           streak_len=(5, 12))
     raw = ph.make(spec)
