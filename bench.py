@@ -266,7 +266,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the filter shards by z-slabs with a "
-              "halo exchange per iteration, which this build does not run).", file=sys.stderr)
+              "halo exchange per iteration, which is not implemented).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -4,9 +4,10 @@
  * Restates protocol::DenoiseParameters of the reference
  * (iterNLM_v0.3/denoise_parameters.h:9-74) with the same namespace, field
  * names, nesting and defaults, so code written against the reference compiles
- * unchanged against this build.  Three fields are this build's own and sit in
- * the GPU block: the arithmetic form, the weight function and the sample
- * seed of the noise estimator, and a device memory limit.
+ * unchanged against these headers.  Four fields are additions of this
+ * library and sit in the GPU block: the arithmetic form, the weight
+ * function, the sample seed of the noise estimator and a device memory
+ * limit.
  */
 #ifndef INLM_DENOISE_PARAMETERS_H
 #define INLM_DENOISE_PARAMETERS_H

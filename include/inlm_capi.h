@@ -81,7 +81,7 @@ typedef enum {
 typedef enum { INLM_WEIGHT_PADE = 0, INLM_WEIGHT_EXP = 1 } inlm_weight_fn;
 
 /* POD mirror of protocol::DenoiseParameters (denoise_parameters.h:9-74),
- * restricted to the fields the hot path reads, plus the build's own
+ * restricted to the fields the hot path reads, plus this library's own
  * precision / weight-function / sampler-seed choices. */
 typedef struct inlm_params {
     float alpha;                   /* 0.5 */

@@ -30,7 +30,7 @@ const char *kUsage =
     "            -nsamples/-noise_samples N  -noisepatch N  -noiseshift F  -mask/-mask_diameter N\n"
     "            --continuous  --preserve_detail\n"
     "  hardware: -gpu0/-device N  -gpus N (one GPU is used)  --blocks  --noblocks  --verbose/--v\n"
-    "  this build: -precision fp32|fp32_fast|fp16|bf16|fp16_fp8  -weight pade|exp  -seed N  --print-params\n";
+    "  additions:  -precision fp32|fp32_fast|fp16|bf16|fp16_fp8  -weight pade|exp  -seed N  --print-params\n";
 
 bool is(const char *a, const char *b) { return std::strcmp(a, b) == 0; }
 
@@ -154,7 +154,7 @@ int main(int argc, char **argv)
             p.stds_from_mean = 4.f;
             p.maxiterations = 6;
         }
-        // this build
+        // flags of this library, not in the reference
         else if (is(a, "-precision")) {
             NEED(a);
             if (is(v, "fp32")) p.precision = INLM_PREC_FP32;
