@@ -1652,13 +1652,22 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         for (int i = 0; i < 4; i++) rA.v[h][i] = rB.v[h][i] = TS(0.f);
     }
     constexpr int kBlock = 62;
-    int linv = 0, mvv = 0, kb = 0;
+    // per lane of a 62-step block: offset kb + lane packed as lin * 4 + moves;
+    // a step reads the entry of offset k+2 and carries those of k and k+1
+    int pkv = 0, kb = 0;
+    auto packed_at = [&](int k) __attribute__((always_inline)) {
+        return lin_at(k) * 4 + (kMove && k < nS ? moves[k] : 0);
+    };
+    int pc0 = packed_at(0), pc1 = packed_at(1);
     auto step = [&](int k, const _Float16 *rd, _Float16 *st, DVals &dout, RVals &rout, const DVals &din,
                     const RVals &rin) __attribute__((always_inline)) {
-        const int l0 = __builtin_amdgcn_readlane(linv, k - kb);
-        const int l2 = __builtin_amdgcn_readlane(linv, k + 2 - kb);
-        const int m1 = kMove ? __builtin_amdgcn_readlane(mvv, k + 1 - kb) : 0;
-        const int m2 = kMove ? __builtin_amdgcn_readlane(mvv, k + 2 - kb) : 0;
+        const int p2 = __builtin_amdgcn_readlane(pkv, k + 2 - kb);
+        const int l0 = pc0 >> 2;
+        const int l2 = p2 >> 2;
+        const int m1 = pc1 & 3;
+        const int m2 = p2 & 3;
+        pc0 = pc1;
+        pc1 = p2;
         load_r(rout, l0);
         DVals acc;
 #pragma unroll
@@ -1705,8 +1714,7 @@ __global__ __launch_bounds__(NTM) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         // spilled) through the inner loop; volatile, so it is not hoisted
         int ln;
         asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(ln));
-        linv = lin_at(kb + ln);
-        if constexpr (kMove) mvv = kb + ln < nS ? moves[kb + ln] : 0;
+        pkv = packed_at(kb + ln);
         const int ke = kb + kBlock < nS ? kb + kBlock : nS;
         for (int k = kb; k < ke; k += 2) {
             step(k, lds[0], lds[1], dA, rA, dB, rB);
@@ -2021,9 +2029,11 @@ hipError_t launch_mfma_t(const void *Rs, const void *Ps, size_t elems, long shif
         else fp8_tables_plan<R, false>(codes, a);
     }
     // the block form: full search at patch radius 2, 16-bit storage, FP16
-    // planes, Pade weights
+    // planes, Pade weights; it packs lin * 4 + moves into one int, so |lin|
+    // stays below 2^29 (the search space is symmetric: the largest lin is
+    // -min_lin); beyond that the row form runs
     if constexpr (R == 2 && sizeof(TS) == 2 && std::is_same<TD, _Float16>::value && WFN == 0) {
-        if (!ol && !row_form) {
+        if (!ol && !row_form && -(long)min_lin < (1L << 29)) {
             a.ntx = (g.nx + b4::TX - 1) / b4::TX;
             a.nty = (g.ny + b4::TY - 1) / b4::TY;
             a.ntz = (g.nz + b4::TZ - 1) / b4::TZ;
